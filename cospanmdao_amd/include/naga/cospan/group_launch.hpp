@@ -1,0 +1,213 @@
+#pragma once
+
+// =============================================================================
+// naga/cospan/group_launch.hpp — one launch of a fused group (host side)
+// =============================================================================
+//
+// The host half of the reference's GroupLauncher::launch (cospanmdao/include/naga/cospan/component.hpp:779-970): from the
+// stages' launch facts (units, work-group shape, LDS bytes) and the segment layout it builds the geometry (one entry per
+// stage, the segment table), the physical grid of the range_split strategy (component.hpp:936-954), the LDS block
+// (partitioned: a slice per stage; shared: one block), the ticket state when the group waits, and submits ONE kernel
+// through the C boundary (cs_hip_launch). Checks with the reference's messages: one work-group shape per group
+// (component.hpp:815-829), the LDS limit (:880-890).
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cospan_hip.h"
+#include "naga/cospan/kernel.hpp"
+
+namespace naga::cospan::detail {
+
+/// A C-boundary status as an exception (the reference's exception types: std::invalid_argument for a bad argument,
+/// std::logic_error for a call out of order, std::runtime_error for the runtime).
+inline void hip_check(int status, const std::string& what) {
+    if (status == CS_HIP_OK) return;
+    char msg[512];
+    cs_hip_last_error(msg, sizeof msg);
+    const std::string text = "naga::cospan: " + what + ": " + msg;
+    if (status == CS_HIP_ERR_INVALID) throw std::invalid_argument(text);
+    if (status == CS_HIP_ERR_STATE) throw std::logic_error(text);
+    throw std::runtime_error(text);
+}
+
+/// What a launcher needs of one stage.
+struct StagePlan {
+    std::uint64_t units[3] = {1, 1, 1};  ///< work-groups (group-range) or indices (index-range), leading 1s
+    std::uint32_t wg[3]    = {1, 1, 1};  ///< work-group shape, physical order (d0, d1, d2), leading 1s
+    std::size_t   local_bytes = 0;
+    std::size_t   local_align = 1;
+    bool          group_range = true;
+    std::string   who;  ///< "'path'" for messages
+};
+
+/// What a launcher needs of one segment: its stages [first, first + count), options, the boundary before it.
+struct SegmentPlan {
+    int       first = 0;
+    int       count = 0;
+    RangeMode range = RangeMode::own;
+    LocalMode local = LocalMode::partitioned;
+    Boundary  boundary = Boundary::none;
+};
+
+/// The ticket state of a waiting group: device words, zeroed once at allocation; every launch leaves them zeroed.
+class TicketState {
+public:
+    TicketState() = default;
+    TicketState(const TicketState&)            = delete;
+    TicketState& operator=(const TicketState&) = delete;
+    ~TicketState() {
+        if (ptr_) cs_hip_free(ptr_);
+    }
+    std::uint32_t* ensure(std::size_t words, cs_hip_stream s) {
+        if (words > words_) {
+            if (ptr_) {
+                hip_check(cs_hip_stream_synchronize(s), "ticket state");
+                hip_check(cs_hip_free(ptr_), "ticket state");
+                ptr_ = nullptr;
+            }
+            const std::size_t w = words < 1024 ? 1024 : words;
+            void*             p = nullptr;
+            hip_check(cs_hip_malloc(&p, w * 4), "ticket state");
+            hip_check(cs_hip_fill_u32_async(p, 0u, w, s), "ticket state");
+            ptr_   = static_cast<std::uint32_t*>(p);
+            words_ = w;
+        }
+        return ptr_;
+    }
+
+private:
+    std::uint32_t* ptr_   = nullptr;
+    std::size_t    words_ = 0;
+};
+
+/// Builds the geometry of a group and launches its kernel.
+template <class... X>
+class GroupLauncher {
+public:
+    static constexpr int N = sizeof...(X);
+    static_assert(N >= 1 && N <= kMaxStages, "cospan: a fused group has 1..16 stages");
+    static constexpr bool kGroupRange = (GroupRangeStage<X> && ...);
+    static constexpr bool kIndexRange = (IndexRangeStage<X> && ...);
+    static_assert(kGroupRange || kIndexRange,
+                  "cospan: the stages of one fused group are all group-range or all index-range components");
+    static constexpr int kMaxThreads = std::min({max_wg_of<X>::value...});
+
+    /// The host stub of this group's kernel (what the process-wide kernel cache of the reference keys by stage sequence:
+    /// here the C++ type system does it, one instantiation per sequence).
+    static const void* kernel() {
+        if constexpr (kGroupRange)
+            return reinterpret_cast<const void*>(&group_kernel<kMaxThreads, X...>);
+        else
+            return reinterpret_cast<const void*>(&index_kernel<X...>);
+    }
+
+    /// Launches the group on `s`. `split`: the range_split strategy; `waits`: group_waits_enabled or a global_barrier.
+    void launch(Split split, bool waits, const std::vector<SegmentPlan>& segs, const StagePlan (&stages)[N], const ArgPack<X...>& args,
+                cs_hip_stream s, const std::string& group_name) {
+        GroupArgs<X...> ga{};
+        ga.args                 = args;
+        Geometry<N>&  geo       = ga.geo;
+        const auto    nseg      = static_cast<std::uint32_t>(segs.size());
+        if (nseg == 0 || nseg > static_cast<std::uint32_t>(N)) throw std::logic_error("naga::cospan: '" + group_name + "': bad segment layout");
+        geo.segments = nseg;
+        geo.split    = static_cast<std::uint32_t>(split);
+
+        // the one work-group shape of the group (group-range stages)
+        std::uint32_t wg[3] = {1, 1, 1};
+        if constexpr (kGroupRange) {
+            bool first = true;
+            for (int k = 0; k < N; ++k) {
+                if (first) {
+                    for (int d = 0; d < 3; ++d) wg[d] = stages[k].wg[d];
+                    first = false;
+                } else if (stages[k].wg[0] != wg[0] || stages[k].wg[1] != wg[1] || stages[k].wg[2] != wg[2]) {
+                    throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': stage " + stages[k].who +
+                                                " has a different work-group shape; all stages of a group share one");
+                }
+            }
+            if (static_cast<std::uint64_t>(wg[0]) * wg[1] * wg[2] > static_cast<std::uint64_t>(kMaxThreads))
+                throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': a work-group of " +
+                                            std::to_string(wg[0] * wg[1] * wg[2]) + " work-items exceeds the stages' bound of " +
+                                            std::to_string(kMaxThreads));
+        }
+
+        // segment table, LDS layout
+        std::uint64_t first_ticket = 0, offset = 0, lds = 0;
+        std::uint64_t maxr[3]      = {0, 0, 0};
+        for (std::uint32_t si = 0; si < nseg; ++si) {
+            const SegmentPlan& sp = segs[si];
+            SegmentInfo&       info = geo.seg[si];
+            std::uint64_t      seg_lds = 0;
+            for (int k = sp.first; k < sp.first + sp.count; ++k) {
+                for (int d = 0; d < 3; ++d)
+                    if (stages[k].units[d] > info.range[d]) info.range[d] = stages[k].units[d];
+                StageGeometry& g = geo.stage[k];
+                g.segment        = si;
+                g.range_mode     = static_cast<std::uint32_t>(sp.range);
+                if (sp.local == LocalMode::shared) {
+                    g.local_offset = 0;
+                    if (stages[k].local_bytes > seg_lds) seg_lds = stages[k].local_bytes;
+                } else {
+                    seg_lds        = align_up(seg_lds, stages[k].local_align);
+                    g.local_offset = static_cast<std::uint32_t>(seg_lds);
+                    seg_lds += stages[k].local_bytes;
+                }
+                g.local_bytes = static_cast<std::uint32_t>(stages[k].local_bytes);
+            }
+            info.count    = info.range[0] * info.range[1] * info.range[2];
+            info.first    = first_ticket;
+            info.offset   = offset;
+            info.boundary = static_cast<std::uint64_t>(sp.boundary);
+            first_ticket += info.count;
+            offset += info.range[2];
+            for (int d = 0; d < 3; ++d)
+                if (info.range[d] > maxr[d]) maxr[d] = info.range[d];
+            if (seg_lds > lds) lds = seg_lds;
+        }
+        geo.total_groups = first_ticket;
+        if (lds > 160u * 1024u)
+            throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': " + std::to_string(lds) +
+                                        " bytes of local memory exceed the 163840 of a CDNA4 work-group");
+        if (geo.total_groups == 0) return;  // nothing to run
+
+        // physical grid (x is physical dimension 2)
+        std::uint32_t grid[3] = {1, 1, 1};
+        if constexpr (kIndexRange) {
+            grid[0] = static_cast<std::uint32_t>((geo.total_groups + 255) / 256);
+            wg[2]   = 256;
+            if (waits) throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': index-range stages cannot wait");
+            geo.split = static_cast<std::uint32_t>(Split::flatten);
+        } else if (waits || split == Split::flatten) {
+            if (geo.total_groups > 0x7fffffffull)
+                throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': too many work-groups for one launch");
+            grid[0] = static_cast<std::uint32_t>(geo.total_groups);
+            if (waits) geo.split = static_cast<std::uint32_t>(Split::flatten);
+        } else if (split == Split::pad) {
+            grid[0] = static_cast<std::uint32_t>(offset);
+            grid[1] = static_cast<std::uint32_t>(maxr[1]);
+            grid[2] = static_cast<std::uint32_t>(maxr[0]);
+        } else {
+            if (maxr[0] > 1) throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': range_split::padded needs stages of rank 2 or lower");
+            grid[0] = static_cast<std::uint32_t>(maxr[2]);
+            grid[1] = static_cast<std::uint32_t>(maxr[1]);
+            grid[2] = nseg;
+        }
+
+        if (waits) geo.ticket_state = tickets_.ensure(ticket_words(nseg, geo.total_groups), s);
+
+        const std::uint32_t block[3] = {wg[2], wg[1], wg[0]};
+        void*               kargs[]  = {&ga};
+        static_assert(sizeof(GroupArgs<X...>) <= 4096, "cospan: a fused group's arguments exceed the 4 KiB kernel-argument limit");
+        hip_check(cs_hip_launch(kernel(), grid, block, static_cast<std::size_t>(lds), kargs, s), "launch of '" + group_name + "'");
+    }
+
+private:
+    TicketState tickets_;
+};
+
+}  // namespace naga::cospan::detail

@@ -1,0 +1,735 @@
+#pragma once
+
+// =============================================================================
+// naga/cospan/kernel.hpp — the fused-group kernel: stage ABI, tickets, GroupItem, LocalMem, group functions
+// =============================================================================
+//
+// The reference assembles a fused group at RUN TIME: one generic kernel calls a placeholder that AdaptiveCpp's JIT
+// replaces with an indexed call sequence of per-stage functions (cospanmdao/include/naga/cospan/kernel.hpp:911-1004).
+// Here the assembly happens at COMPILE TIME: a fused group's stage types are known to the C++ type system
+// (FusedGroup<..., Items...>), so hipcc instantiates ONE kernel per group,
+//
+//     template <int MaxThreads, class... X> __global__ void group_kernel(GroupArgs<X...>)
+//
+// whose body locates the work-group (ticket or hardware id -> segment and group coordinate), runs every stage
+// X_0 .. X_n-1 in order as a fold over an index sequence (each stage_run<X_k> guarded by its segment and own range, as
+// stage_fn<KB> is in the reference, kernel.hpp:919-950) and signals completion. There are no indirect calls and no
+// device-side argument arrays: the geometry (one entry per stage, the segment table) and every stage's arguments
+// (its Inputs and Outputs views, its units, its LocalMem shapes) travel as kernel arguments, so a launch needs no
+// host-to-device upload and can be captured in a graph as it is.
+//
+// Ticket mode (a group with cs::group_waits_enabled or a cs::global_barrier): the state is a u32 array
+// [ticket counter, finished counter, S segment-done counters, T per-group done flags]. Each work-group's leader takes the
+// next ticket (so a group only ever waits on groups that have started: no co-residency needed, reference
+// kernel.hpp:197-205), waits are agent-scope acquire spins by the leader plus a barrier, completion is a barrier, a
+// release fence and the leader's release atomics. The last group to finish zeroes the whole state, so the next launch
+// (or graph replay) starts from zero without a memset.
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <limits>
+#include <type_traits>
+#include <utility>
+
+#include "naga/cospan/device.hpp"
+#include "naga/cospan/dtype.hpp"
+#include "naga/cospan/vocab.hpp"
+#include "naga/cospan/wrappers.hpp"
+
+namespace naga::cospan {
+
+/// Local memory of a segment: partitioned (default, a slice per stage) or shared (one block, the max).
+namespace local_mem {
+struct partitioned {};
+struct shared {};
+}  // namespace local_mem
+
+/// Range of a segment: own (default, a stage runs for its own units only) or whole (every stage runs every unit).
+namespace range_mode {
+struct own {};
+struct whole {};
+}  // namespace range_mode
+
+namespace detail {
+
+enum class LocalMode : std::uint8_t { partitioned = 0, shared = 1 };
+enum class RangeMode : std::uint8_t { own = 0, whole = 1 };
+enum class Boundary : std::uint8_t { none = 0, global_barrier = 1 };
+enum class Split : std::uint8_t { flatten = 0, pad = 1, padded = 2 };
+
+inline constexpr int kMaxStages = 16;
+
+/// One entry per stage (reference kernel.hpp:109-120).
+struct StageGeometry {
+    std::uint32_t segment      = 0;
+    std::uint32_t local_offset = 0;  ///< bytes into the launch's LDS block
+    std::uint32_t local_bytes  = 0;
+    std::uint32_t range_mode   = 0;
+};
+
+/// One entry per segment (reference kernel.hpp:127-133): its block of groups as a rank-3 range, the logical count, the first
+/// ticket, the offset along the last dimension (pad split) and the boundary before it.
+struct SegmentInfo {
+    std::uint64_t range[3] = {0, 0, 0};
+    std::uint64_t count    = 0;
+    std::uint64_t first    = 0;
+    std::uint64_t offset   = 0;
+    std::uint64_t boundary = 0;
+};
+
+template <int N>
+struct Geometry {
+    std::uint32_t* ticket_state = nullptr;  ///< null: no ticket mode
+    std::uint64_t  total_groups = 0;
+    std::uint32_t  segments     = 0;
+    std::uint32_t  split        = 0;
+    StageGeometry  stage[N];
+    SegmentInfo    seg[N];
+};
+
+// ---- rank-R values in the trailing physical dimensions ------------------------------------------------------------------
+template <int R>
+CS_HD inline id<R> tail_id(std::size_t c0, std::size_t c1, std::size_t c2) {
+    if constexpr (R == 1)
+        return id<1>{c2};
+    else if constexpr (R == 2)
+        return id<2>{c1, c2};
+    else
+        return id<3>{c0, c1, c2};
+}
+template <int R>
+CS_HD inline range<R> tail_range(std::size_t c0, std::size_t c1, std::size_t c2) {
+    if constexpr (R == 1)
+        return range<1>{c2};
+    else if constexpr (R == 2)
+        return range<2>{c1, c2};
+    else
+        return range<3>{c0, c1, c2};
+}
+template <int R>
+CS_HD inline range<3> lift_range(const range<R>& r) {
+    if constexpr (R == 1)
+        return range<3>{1, 1, r[0]};
+    else if constexpr (R == 2)
+        return range<3>{1, r[0], r[1]};
+    else
+        return r;
+}
+template <int R>
+CS_HD inline id<3> lift_id(const id<R>& r) {
+    if constexpr (R == 1)
+        return id<3>{0, 0, r[0]};
+    else if constexpr (R == 2)
+        return id<3>{0, r[0], r[1]};
+    else
+        return r;
+}
+
+/// Where a work-group sits logically.
+struct LogicalGroup {
+    std::uint32_t segment = 0;
+    std::uint64_t coord[3] = {0, 0, 0};
+    bool          valid   = false;
+};
+
+__device__ inline std::uint64_t segment_linear(const SegmentInfo& s, const std::uint64_t* c) {
+    return (c[0] * s.range[1] + c[1]) * s.range[2] + c[2];
+}
+
+/// Group number `lin` (a ticket or the flattened hardware id) -> segment and coordinate, segment-major then row-major
+/// (reference kernel.hpp:246-260).
+__device__ inline LogicalGroup locate_linear(const SegmentInfo* tbl, std::uint32_t segments, std::uint64_t lin, std::uint64_t total) {
+    LogicalGroup r;
+    if (lin >= total) return r;
+    std::uint32_t s = 0;
+    while (s + 1 < segments && lin >= tbl[s].first + tbl[s].count) ++s;
+    std::uint64_t k = lin - tbl[s].first;
+    r.segment       = s;
+    r.coord[2]      = k % tbl[s].range[2];
+    k /= tbl[s].range[2];
+    r.coord[1] = k % tbl[s].range[1];
+    r.coord[0] = k / tbl[s].range[1];
+    r.valid    = true;
+    return r;
+}
+
+/// Hardware-direct mapping (no tickets), per split strategy (reference kernel.hpp:266-288). Physical dimension 2 is
+/// HIP's x, 1 is y, 0 is z.
+__device__ inline LogicalGroup locate_physical(std::uint32_t split, const SegmentInfo* tbl, std::uint32_t segments,
+                                               std::uint64_t total) {
+    const std::uint64_t d0 = blockIdx.z, d1 = blockIdx.y, d2 = blockIdx.x;
+    if (split == static_cast<std::uint32_t>(Split::flatten)) return locate_linear(tbl, segments, d2, total);
+    LogicalGroup r;
+    if (split == static_cast<std::uint32_t>(Split::pad)) {
+        for (std::uint32_t s = 0; s < segments; ++s) {
+            if (d2 >= tbl[s].offset && d2 < tbl[s].offset + tbl[s].range[2]) {
+                if (d0 < tbl[s].range[0] && d1 < tbl[s].range[1]) {
+                    r.segment  = s;
+                    r.coord[0] = d0;
+                    r.coord[1] = d1;
+                    r.coord[2] = d2 - tbl[s].offset;
+                    r.valid    = true;
+                }
+                break;
+            }
+        }
+    } else {
+        if (d0 < segments && d1 < tbl[d0].range[1] && d2 < tbl[d0].range[2]) {
+            r.segment  = static_cast<std::uint32_t>(d0);
+            r.coord[0] = 0;
+            r.coord[1] = d1;
+            r.coord[2] = d2;
+            r.valid    = true;
+        }
+    }
+    return r;
+}
+
+// ---- ticket state ---------------------------------------------------------------------------------------------------
+// [0] ticket counter, [1] finished counter, [2, 2+S) per-segment finished counts, [2+S, 2+S+T) per-ticket done flags.
+inline constexpr std::size_t kTicketHeader = 2;
+CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return kTicketHeader + segments + total; }
+
+__device__ inline std::uint32_t acquire_load(std::uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+
+/// Waits only ever target groups with earlier tickets, which have started and finish without waiting on later ones, so
+/// they end; the bound (about a second) only guarantees that corrupted ticket state cannot hang the device.
+inline constexpr std::uint32_t kSpinLimit = 1u << 24;
+
+/// COLLECTIVE: wait for group `c` of segment `s` (reference kernel.hpp:215-224).
+__device__ inline void ticket_wait_group(std::uint32_t* ts, std::uint32_t segments, const SegmentInfo* tbl, std::uint32_t s,
+                                         const std::uint64_t* c) {
+    if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
+        const std::uint64_t t = tbl[s].first + segment_linear(tbl[s], c);
+        for (std::uint32_t spin = 0; acquire_load(&ts[kTicketHeader + segments + t]) == 0 && spin < kSpinLimit; ++spin)
+            __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+/// COLLECTIVE: wait until every group of segments 0 .. upto-1 has finished (reference kernel.hpp:227-236).
+__device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo* tbl, std::uint32_t upto) {
+    if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
+        for (std::uint32_t i = 0; i < upto; ++i)
+            for (std::uint32_t spin = 0;
+                 static_cast<std::uint64_t>(acquire_load(&ts[kTicketHeader + i])) != tbl[i].count && spin < kSpinLimit; ++spin)
+                __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ inline std::uint32_t local_linear_id() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
+__device__ inline std::uint32_t local_linear_range() { return blockDim.x * blockDim.y * blockDim.z; }
+
+struct ItemFixup;
+}  // namespace detail
+
+template <int R>
+struct GroupItem;
+template <class... Arrays>
+struct LocalMem;
+
+// -----------------------------------------------------------------------------
+// Group<R>: the work-group a stage belongs to, with rank-R ids (reference kernel.hpp:317-360)
+// -----------------------------------------------------------------------------
+template <int R>
+struct Group {
+    __device__ id<R>       get_group_id() const { return id_; }
+    __device__ std::size_t get_group_id(int d) const { return id_[d]; }
+    __device__ range<R>    get_group_range() const { return range_; }
+    __device__ std::size_t get_group_range(int d) const { return range_[d]; }
+    __device__ std::size_t get_group_linear_id() const {
+        std::size_t l = 0;
+        for (int d = 0; d < R; ++d) l = l * range_[d] + id_[d];
+        return l;
+    }
+    __device__ std::size_t get_group_linear_range() const { return range_.size(); }
+    __device__ id<R>       get_local_id() const { return detail::tail_id<R>(threadIdx.z, threadIdx.y, threadIdx.x); }
+    __device__ std::size_t get_local_id(int d) const { return get_local_id()[d]; }
+    __device__ range<R>    get_local_range() const { return detail::tail_range<R>(blockDim.z, blockDim.y, blockDim.x); }
+    __device__ std::size_t get_local_range(int d) const { return get_local_range()[d]; }
+    __device__ std::size_t get_local_linear_id() const { return detail::local_linear_id(); }
+    __device__ std::size_t get_local_linear_range() const { return detail::local_linear_range(); }
+    __device__ bool        leader() const { return detail::local_linear_id() == 0; }
+
+    id<R>    id_;
+    range<R> range_;
+};
+
+// -----------------------------------------------------------------------------
+// GroupItem<R>: what a group-range body receives (reference kernel.hpp:364-446)
+// -----------------------------------------------------------------------------
+template <int R>
+struct GroupItem {
+    __device__ id<R>       get_local_id() const { return detail::tail_id<R>(threadIdx.z, threadIdx.y, threadIdx.x); }
+    __device__ std::size_t get_local_id(int d) const { return get_local_id()[d]; }
+    __device__ range<R>    get_local_range() const { return detail::tail_range<R>(blockDim.z, blockDim.y, blockDim.x); }
+    __device__ std::size_t get_local_range(int d) const { return get_local_range()[d]; }
+    __device__ std::size_t get_local_linear_id() const { return detail::local_linear_id(); }
+
+    __device__ id<R>       get_group_id() const { return group_id_; }
+    __device__ std::size_t get_group(int d) const { return group_id_[d]; }
+    __device__ range<R>    get_group_range() const { return group_range_; }
+    __device__ std::size_t get_group_range(int d) const { return group_range_[d]; }
+    __device__ std::size_t get_group_linear_id() const {
+        std::size_t l = 0;
+        for (int d = 0; d < R; ++d) l = l * group_range_[d] + group_id_[d];
+        return l;
+    }
+    __device__ id<R> get_global_id() const {
+        id<R>          g;
+        const range<R> lr = get_local_range();
+        const id<R>    li = get_local_id();
+        for (int d = 0; d < R; ++d) g[d] = group_id_[d] * lr[d] + li[d];
+        return g;
+    }
+    __device__ std::size_t get_global_id(int d) const { return get_global_id()[d]; }
+    __device__ range<R>    get_global_range() const {
+        range<R>       g;
+        const range<R> lr = get_local_range();
+        for (int d = 0; d < R; ++d) g[d] = group_range_[d] * lr[d];
+        return g;
+    }
+    __device__ std::size_t get_global_linear_id() const {
+        const id<R>    g = get_global_id();
+        const range<R> r = get_global_range();
+        std::size_t    l = 0;
+        for (int d = 0; d < R; ++d) l = l * r[d] + g[d];
+        return l;
+    }
+
+    __device__ std::size_t get_segment() const { return segment_; }
+    __device__ std::size_t get_segment_count() const { return segments_; }
+    __device__ range<R>    get_segment_group_count(std::size_t s) const {
+        return detail::tail_range<R>(table_[s].range[0], table_[s].range[1], table_[s].range[2]);
+    }
+
+    /// COLLECTIVE: wait until group `g` of the EARLIER segment `s` has finished all its stages (ticket mode only).
+    __device__ void wait_group(std::size_t s, const id<R>& g) const {
+        const id<3>         l = detail::lift_id<R>(g);
+        const std::uint64_t c[3] = {l[0], l[1], l[2]};
+        detail::ticket_wait_group(ticket_state_, segments_, table_, static_cast<std::uint32_t>(s), c);
+    }
+    /// COLLECTIVE: wait until every group of every earlier segment has finished (what cs::global_barrier() does).
+    __device__ void wait_earlier_segments() const { detail::ticket_wait_segments(ticket_state_, table_, segment_); }
+
+    __device__ Group<R> get_group() const { return Group<R>{group_id_, group_range_}; }
+    __device__ void     barrier() const { __syncthreads(); }
+
+    id<R>                      group_id_;
+    range<R>                   group_range_;
+    std::uint32_t              segment_;
+    std::uint32_t              segments_;
+    const detail::SegmentInfo* table_;
+    std::uint32_t*             ticket_state_;
+    std::byte*                 base_;  ///< this stage's slice of the launch's LDS block
+};
+
+// -----------------------------------------------------------------------------
+// LocalMem (reference kernel.hpp:515-570)
+// -----------------------------------------------------------------------------
+namespace detail {
+template <class T>
+struct is_local_array : std::false_type {};
+template <DType DT, int R>
+struct is_local_array<LocalArray<DT, R>> : std::true_type {};
+
+CS_HD constexpr std::size_t align_up(std::size_t v, std::size_t a) { return (v + a - 1) / a * a; }
+
+template <class... Vs>
+struct LocalViews;
+template <class A>
+struct LocalViews<A> { A v0; };
+template <class A, class B>
+struct LocalViews<A, B> { A v0; B v1; };
+template <class A, class B, class C>
+struct LocalViews<A, B, C> { A v0; B v1; C v2; };
+template <class A, class B, class C, class D>
+struct LocalViews<A, B, C, D> { A v0; B v1; C v2; D v3; };
+template <class A, class B, class C, class D, class E>
+struct LocalViews<A, B, C, D, E> { A v0; B v1; C v2; D v3; E v4; };
+template <class A, class B, class C, class D, class E, class F>
+struct LocalViews<A, B, C, D, E, F> { A v0; B v1; C v2; D v3; E v4; F v5; };
+
+template <std::size_t I, class... As>
+struct nth_array;
+template <class A, class... As>
+struct nth_array<0, A, As...> { using type = A; };
+template <std::size_t I, class A, class... As>
+struct nth_array<I, A, As...> : nth_array<I - 1, As...> {};
+}  // namespace detail
+
+template <class... Arrays>
+struct LocalMem {
+    static_assert(sizeof...(Arrays) >= 1 && sizeof...(Arrays) <= 6, "cospan: a LocalMem holds one to six arrays");
+    static_assert((detail::is_local_array<Arrays>::value && ...), "cospan: a LocalMem holds cs::LocalArray types");
+    static constexpr std::size_t count = sizeof...(Arrays);
+    template <std::size_t I>
+    using array_t = typename detail::nth_array<I, Arrays...>::type;
+
+    CS_HD LocalMem() {
+        for (std::size_t i = 0; i < count; ++i) shapes_[i] = range<3>{0, 0, 0};
+    }
+    CS_HD explicit LocalMem(range<Arrays::rank>... shapes) : shapes_{detail::lift_range<Arrays::rank>(shapes)...} {}
+
+    template <std::size_t I>
+    CS_HD range<array_t<I>::rank> shape() const {
+        const range<3>& s = shapes_[I];
+        return detail::tail_range<array_t<I>::rank>(s[0], s[1], s[2]);
+    }
+
+    /// The bytes of the stage's slice: each array at a multiple of its element alignment.
+    CS_HD std::size_t bytes() const {
+        std::size_t cur = 0;
+        [&]<std::size_t... I>(std::index_sequence<I...>) {
+            ((cur = detail::align_up(cur, alignof(typename array_t<I>::value_type)) +
+                    shape<I>().size() * sizeof(typename array_t<I>::value_type)),
+             ...);
+        }(std::make_index_sequence<count>{});
+        return cur;
+    }
+
+    template <int R>
+    __device__ auto get(const GroupItem<R>& it) const {
+        std::byte* const base = it.base_;
+        return [&]<std::size_t... I>(std::index_sequence<I...>) {
+            std::size_t off[count] = {};
+            std::size_t cur        = 0;
+            ((cur = detail::align_up(cur, alignof(typename array_t<I>::value_type)), off[I] = cur,
+              cur += shape<I>().size() * sizeof(typename array_t<I>::value_type)),
+             ...);
+            return detail::LocalViews<typename array_t<I>::view_type...>{typename array_t<I>::view_type{
+                reinterpret_cast<typename array_t<I>::value_type*>(base + off[I]), shape<I>()}...};
+        }(std::make_index_sequence<count>{});
+    }
+
+    range<3> shapes_[count];
+};
+
+// -----------------------------------------------------------------------------
+// Group functions (reference kernel.hpp:581-694), over Group<R>
+// -----------------------------------------------------------------------------
+namespace detail {
+template <class Op, class T>
+CS_HD constexpr T identity_of() {
+    if constexpr (std::is_same_v<Op, minimum<T>>)
+        return std::numeric_limits<T>::has_infinity ? std::numeric_limits<T>::infinity() : std::numeric_limits<T>::max();
+    else if constexpr (std::is_same_v<Op, maximum<T>>)
+        return std::numeric_limits<T>::has_infinity ? -std::numeric_limits<T>::infinity() : std::numeric_limits<T>::lowest();
+    else if constexpr (std::is_same_v<Op, multiplies<T>>)
+        return T(1);
+    else if constexpr (std::is_same_v<Op, bit_and<T>>)
+        return static_cast<T>(~T(0));
+    else
+        return T(0);
+}
+}  // namespace detail
+
+template <int R>
+__device__ inline void group_barrier(const Group<R>&, memory_scope = memory_scope::work_group) {
+    __syncthreads();
+}
+
+template <int R, class T>
+__device__ inline T group_broadcast(const Group<R>&, T x, std::size_t local_linear_id = 0) {
+    __shared__ T slot;
+    if (detail::local_linear_id() == local_linear_id) slot = x;
+    __syncthreads();
+    const T v = slot;
+    __syncthreads();
+    return v;
+}
+
+template <int R>
+__device__ inline bool any_of_group(const Group<R>&, bool pred) {
+    return __syncthreads_or(pred ? 1 : 0) != 0;
+}
+template <int R>
+__device__ inline bool all_of_group(const Group<R>&, bool pred) {
+    return __syncthreads_and(pred ? 1 : 0) != 0;
+}
+template <int R>
+__device__ inline bool none_of_group(const Group<R>& g, bool pred) {
+    return !any_of_group(g, pred);
+}
+
+/// Every work-item gets the reduction: wave butterflies then a pairwise tree over the waves (deterministic).
+template <int R, class T, class Op>
+__device__ inline T reduce_over_group(const Group<R>&, T x, Op op) {
+    return block::reduce(x, op);
+}
+template <int R, class V, class T, class Op>
+__device__ inline T reduce_over_group(const Group<R>& g, V x, T init, Op op) {
+    return op(init, reduce_over_group(g, static_cast<T>(x), op));
+}
+
+template <int R, class T, class Op>
+__device__ inline T exclusive_scan_over_group(const Group<R>&, T x, Op op) {
+    return block::exclusive_scan(x, op, detail::identity_of<Op, T>());
+}
+template <int R, class V, class T, class Op>
+__device__ inline T exclusive_scan_over_group(const Group<R>&, V x, T init, Op op) {
+    return op(init, block::exclusive_scan(static_cast<T>(x), op, detail::identity_of<Op, T>()));
+}
+template <int R, class T, class Op>
+__device__ inline T inclusive_scan_over_group(const Group<R>&, T x, Op op) {
+    return op(block::exclusive_scan(x, op, detail::identity_of<Op, T>()), x);
+}
+template <int R, class V, class Op, class T>
+__device__ inline T inclusive_scan_over_group(const Group<R>&, V x, Op op, T init) {
+    return op(init, op(block::exclusive_scan(static_cast<T>(x), op, detail::identity_of<Op, T>()), static_cast<T>(x)));
+}
+
+// -----------------------------------------------------------------------------
+// The body concepts, read off X::compute_forward (reference kernel.hpp:712-835, component.hpp:1245-1290)
+// -----------------------------------------------------------------------------
+namespace detail {
+struct None {};
+
+template <class F>
+struct impl_call {
+    static constexpr bool        valid = false;
+    static constexpr std::size_t nargs = 0;
+    using a0 = None;
+    using a3 = None;
+};
+template <class Ret, class... A>
+struct impl_call<Ret (*)(A...)> {
+    static constexpr bool        valid = true;
+    static constexpr std::size_t nargs = sizeof...(A);
+    template <std::size_t I, class Def, class... B>
+    struct at {
+        using type = Def;
+    };
+    template <std::size_t I, class Def, class H, class... B>
+    struct at<I, Def, H, B...> : at<I - 1, Def, B...> {};
+    template <class Def, class H, class... B>
+    struct at<0, Def, H, B...> {
+        using type = H;
+    };
+    using a0 = std::remove_cvref_t<typename at<0, None, A...>::type>;
+    using a3 = std::remove_cvref_t<typename at<3, None, A...>::type>;
+};
+
+template <class X, class = void>
+struct impl_traits : impl_call<void> {};
+template <class X>
+struct impl_traits<X, std::void_t<decltype(&X::compute_forward)>> : impl_call<decltype(&X::compute_forward)> {};
+
+/// kind: 0 index-range (id<R>), 1 group-range (GroupItem<R>).
+template <class T>
+struct first_traits {
+    static constexpr int kind = -1;
+    static constexpr int rank = 0;
+};
+template <int R>
+struct first_traits<id<R>> {
+    static constexpr int kind = 0;
+    static constexpr int rank = R;
+};
+template <int R>
+struct first_traits<GroupItem<R>> {
+    static constexpr int kind = 1;
+    static constexpr int rank = R;
+};
+
+template <class T>
+struct is_local_mem : std::false_type {};
+template <class... As>
+struct is_local_mem<LocalMem<As...>> : std::true_type {};
+
+template <class X>
+inline constexpr int body_kind_v = first_traits<typename impl_traits<X>::a0>::kind;
+template <class X>
+inline constexpr int kernel_rank_v = first_traits<typename impl_traits<X>::a0>::rank;
+template <class X>
+inline constexpr bool has_local_mem_v = impl_traits<X>::nargs == 4 && is_local_mem<typename impl_traits<X>::a3>::value;
+
+template <class X>
+struct local_mem_of {
+    using type = None;
+};
+template <class X>
+    requires has_local_mem_v<X>
+struct local_mem_of<X> {
+    using type = typename impl_traits<X>::a3;
+};
+template <class X>
+using local_mem_t = typename local_mem_of<X>::type;
+
+/// The most work-items a stage's work-group may have, as a compile-time bound (X::max_work_group_size, else 1024): the
+/// fused kernel's launch bound, so that register allocation fits the real work-group.
+template <class X, class = void>
+struct max_wg_of : std::integral_constant<int, 1024> {};
+template <class X>
+struct max_wg_of<X, std::void_t<decltype(X::max_work_group_size)>> : std::integral_constant<int, X::max_work_group_size> {};
+}  // namespace detail
+
+template <class X>
+concept IndexRangeStage = detail::body_kind_v<X> == 0 && detail::impl_traits<X>::nargs == 3;
+template <class X>
+concept GroupRangeStage = detail::body_kind_v<X> == 1 &&
+                          (detail::impl_traits<X>::nargs == 3 || detail::has_local_mem_v<X>);
+template <class X>
+concept KernelStage = (IndexRangeStage<X> || GroupRangeStage<X>) && std::is_trivially_copyable_v<typename X::Inputs> &&
+                      std::is_trivially_copyable_v<typename X::Outputs>;
+
+// -----------------------------------------------------------------------------
+// StageArgs and the argument pack of a launch
+// -----------------------------------------------------------------------------
+namespace detail {
+
+/// Everything a stage reads of its component (reference kernel.hpp:875-886): its units (work-groups, or indices for an
+/// index-range stage) as a rank-3 range with leading 1s, its Inputs and Outputs views, its LocalMem shapes.
+template <class X>
+struct StageArgs {
+    std::uint64_t        units[3] = {0, 0, 0};
+    typename X::Inputs   in;
+    typename X::Outputs  out;
+    local_mem_t<X>       local;
+};
+
+template <class... X>
+struct ArgPack;
+template <>
+struct ArgPack<> {};
+template <class H, class... T>
+struct ArgPack<H, T...> {
+    StageArgs<H>  head;
+    ArgPack<T...> tail;
+};
+
+template <std::size_t I, class... X>
+struct pack_get;
+template <class H, class... T>
+struct pack_get<0, H, T...> {
+    static CS_HD StageArgs<H>&       get(ArgPack<H, T...>& p) { return p.head; }
+    static CS_HD const StageArgs<H>& get(const ArgPack<H, T...>& p) { return p.head; }
+};
+template <std::size_t I, class H, class... T>
+struct pack_get<I, H, T...> {
+    static CS_HD auto&       get(ArgPack<H, T...>& p) { return pack_get<I - 1, T...>::get(p.tail); }
+    static CS_HD const auto& get(const ArgPack<H, T...>& p) { return pack_get<I - 1, T...>::get(p.tail); }
+};
+
+template <class... X>
+struct GroupArgs {
+    Geometry<sizeof...(X)> geo;
+    ArgPack<X...>          args;
+};
+
+struct ItemFixup {
+    template <int R>
+    __device__ static GroupItem<R> make(const StageGeometry& g, const SegmentInfo* tbl, std::uint32_t segments,
+                                        std::uint32_t* ticket_state, const std::uint64_t* coord, const std::uint64_t* own,
+                                        std::byte* lds) {
+        const std::uint64_t* gr = own;
+        if (g.range_mode == static_cast<std::uint32_t>(RangeMode::whole)) gr = tbl[g.segment].range;
+        return GroupItem<R>{tail_id<R>(coord[0], coord[1], coord[2]), tail_range<R>(gr[0], gr[1], gr[2]), g.segment, segments,
+                            tbl, ticket_state, lds + g.local_offset};
+    }
+};
+
+/// Stage K of a group-range launch: segment guard, own-range guard, then the body (reference kernel.hpp:919-950).
+template <class X, int K, int N>
+__device__ inline void stage_run(const Geometry<N>& geo, const StageArgs<X>& a, const LogicalGroup& lg, std::byte* lds) {
+    const StageGeometry& g = geo.stage[K];
+    if (lg.segment != g.segment) return;
+    if (a.units[0] * a.units[1] * a.units[2] == 0) return;
+    if (g.range_mode == static_cast<std::uint32_t>(RangeMode::own))
+        for (int d = 0; d < 3; ++d)
+            if (lg.coord[d] >= a.units[d]) return;
+    constexpr int R = kernel_rank_v<X>;
+    if constexpr (GroupRangeStage<X>) {
+        const GroupItem<R> gi = ItemFixup::make<R>(g, geo.seg, geo.segments, geo.ticket_state, lg.coord, a.units, lds);
+        if constexpr (has_local_mem_v<X>)
+            X::compute_forward(gi, a.in, a.out, a.local);
+        else
+            X::compute_forward(gi, a.in, a.out);
+    } else {
+        X::compute_forward(tail_id<R>(lg.coord[0], lg.coord[1], lg.coord[2]), a.in, a.out);
+    }
+}
+
+template <int N, class... X, std::size_t... K>
+__device__ inline void run_stages(const GroupArgs<X...>& ga, const LogicalGroup& lg, std::byte* lds, std::index_sequence<K...>) {
+    (stage_run<X, static_cast<int>(K), N>(ga.geo, pack_get<K, X...>::get(ga.args), lg, lds), ...);
+}
+
+/// THE fused kernel of a group-range group: one instantiation per distinct stage sequence (reference kernel.hpp:1015-1067).
+template <int MaxThreads, class... X>
+__global__ __launch_bounds__(MaxThreads) void group_kernel(const GroupArgs<X...> ga) {
+    extern __shared__ __attribute__((aligned(16))) std::byte cs_group_lds[];
+    constexpr int        N   = sizeof...(X);
+    const Geometry<N>&   geo = ga.geo;
+    std::uint32_t* const ts  = geo.ticket_state;
+    LogicalGroup         lg;
+    std::uint32_t        ticket = 0;
+    if (ts != nullptr) {
+        __shared__ std::uint32_t s_ticket;
+        if (local_linear_id() == 0) s_ticket = __hip_atomic_fetch_add(&ts[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        ticket = s_ticket;
+        lg     = locate_linear(geo.seg, geo.segments, ticket, geo.total_groups);
+        if (!lg.valid) return;
+        if (geo.seg[lg.segment].boundary == static_cast<std::uint64_t>(Boundary::global_barrier))
+            ticket_wait_segments(ts, geo.seg, lg.segment);
+    } else {
+        lg = locate_physical(geo.split, geo.seg, geo.segments, geo.total_groups);
+        if (!lg.valid) return;
+    }
+
+    run_stages<N>(ga, lg, cs_group_lds, std::make_index_sequence<N>{});
+
+    if (ts != nullptr) {
+        // completion: barrier, release, the group's flag and its segment's count; the last group zeroes the state
+        __syncthreads();
+        __shared__ std::uint32_t s_last;
+        if (local_linear_id() == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&ts[kTicketHeader + geo.segments + ticket], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ts[kTicketHeader + lg.segment], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const std::uint32_t f = __hip_atomic_fetch_add(&ts[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last                = (static_cast<std::uint64_t>(f) + 1 == geo.total_groups) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (s_last) {
+            const std::uint64_t words = ticket_words(geo.segments, geo.total_groups);
+            for (std::uint64_t w = local_linear_id(); w < words; w += local_linear_range())
+                __hip_atomic_store(&ts[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+/// The fused kernel of an index-range group: a flat 1-D launch of indices (reference kernel.hpp:1073-1087).
+template <class X, int K, int N>
+__device__ inline void index_stage_run(const Geometry<N>& geo, const StageArgs<X>& a, const LogicalGroup& lg) {
+    const StageGeometry& g = geo.stage[K];
+    if (lg.segment != g.segment) return;
+    if (g.range_mode == static_cast<std::uint32_t>(RangeMode::own))
+        for (int d = 0; d < 3; ++d)
+            if (lg.coord[d] >= a.units[d]) return;
+    constexpr int R = kernel_rank_v<X>;
+    X::compute_forward(tail_id<R>(lg.coord[0], lg.coord[1], lg.coord[2]), a.in, a.out);
+}
+
+template <int N, class... X, std::size_t... K>
+__device__ inline void run_index_stages(const GroupArgs<X...>& ga, const LogicalGroup& lg, std::index_sequence<K...>) {
+    (index_stage_run<X, static_cast<int>(K), N>(ga.geo, pack_get<K, X...>::get(ga.args), lg), ...);
+}
+
+template <class... X>
+__global__ __launch_bounds__(256) void index_kernel(const GroupArgs<X...> ga) {
+    constexpr int       N   = sizeof...(X);
+    const std::uint64_t lin = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const LogicalGroup  lg  = locate_linear(ga.geo.seg, ga.geo.segments, lin, ga.geo.total_groups);
+    if (!lg.valid) return;
+    run_index_stages<N>(ga, lg, std::make_index_sequence<N>{});
+}
+
+}  // namespace detail
+}  // namespace naga::cospan

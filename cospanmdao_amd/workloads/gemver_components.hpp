@@ -1,0 +1,429 @@
+#pragma once
+
+// =============================================================================
+// cospanmdao_amd/workloads/gemver_components.hpp — the GEMVER components (kernel bodies)
+// =============================================================================
+//
+// GEMVER:  B = A + u1 v1^T + u2 v2^T ;  x = beta * B^T y + z ;  w = alpha * B x.
+//
+// The reference's four components (examples/gemver-chain/components.hpp:75-266), with the floating-point contraction
+// their bodies leave to the compiler fixed here (so the CPU oracle restates it bit for bit):
+//   Rank2Update   B[i][j] = fma(u2_i, v2_j, fma(u1_i, v1_j, A[i][j])), strips of R rows       (components.hpp:85-97)
+//   GemvTPartial  P[c][j] = sum_{k<R} B[cR+k][j] y[cR+k] as an fma chain, k ascending         (components.hpp:142-155)
+//   ColReduce<Z>  x[j] = fma(scale, sum_c P[c][j], z[j]) (Z) or scale * sum (c ascending)      (components.hpp:192-204)
+//   Gemv          out[i] = scale * group_reduce(fma chain over j = l, l + wg, ...)            (components.hpp:237-250)
+//
+// The CDNA4 two-pass form (Â = A + u1 v1^T + u2 v2^T is recomputed in registers on both passes and never stored; A is read
+// twice, with 16-byte vector loads):
+//   AhatTPartial<S>  pass 1: work-group (strip c, column block) computes P[c][j] = sum_{k<R} Â[cR+k][j] y[cR+k] with the
+//                    SAME evaluation order as Rank2Update + GemvTPartial, so x from it equals the components' x bit for bit.
+//   ColReduce<true>  x, as above.
+//   AhatGemv<S, RT>  pass 2: work-group of RT rows; each work-item keeps RT running dots over its columns (column chunks
+//                    of 256 * V, V elements per 16-byte load), then per row a wave butterfly and a pairwise sum of the
+//                    four wave results. Reduced-precision storage accumulates each chunk's partial into a compensated
+//                    (Neumaier) running sum.
+// S is the storage dtype of A: f32, bf16 (exact widening), or e4m3 with a per-row f32 scale (a_ij = e4m3(q_ij) * s_i,
+// widened by the gfx950 conversion instruction). Vectors and all arithmetic are f32; the rank-2 term is never quantised.
+
+#include <cstdint>
+
+#include "naga/cospan/device.hpp"
+#include "naga/cospan/dtype.hpp"
+#include "naga/cospan/kernel.hpp"
+#include "naga/cospan/wrappers.hpp"
+
+namespace gemver {
+
+namespace cs = naga::cospan;
+using cs::DType;
+
+inline constexpr int kWg = 256;  // the work-group size of every GEMVER kernel
+
+__host__ __device__ inline std::uint32_t u32(std::size_t v) { return static_cast<std::uint32_t>(v); }
+
+__device__ inline float rank2(float a, float u1i, float v1j, float u2i, float v2j) { return fmaf(u2i, v2j, fmaf(u1i, v1j, a)); }
+
+// =============================================================================
+// The reference's four components
+// =============================================================================
+
+struct Rank2Update {
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::f32, 2> A;
+        cs::Input<DType::f32, 1> u1, v1, u2, v2;
+        cs::Input<DType::u32, 1> R;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 2> B;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t n = u32(in.A.shape()[1]), R = in.R[std::size_t{0}], total = (n / R) * n;
+        const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        for (std::uint32_t t = u32(it.get_global_id(0)); t < total; t += stride) {
+            const std::uint32_t c = t / n, j = t - c * n;
+            const float         v1j = in.v1[cs::id<1>{j}], v2j = in.v2[cs::id<1>{j}];
+            for (std::uint32_t k = 0; k < R; ++k) {
+                const std::uint32_t i = c * R + k;
+                out.B[cs::id<2>{i, j}] = rank2(in.A[cs::id<2>{i, j}], in.u1[cs::id<1>{i}], v1j, in.u2[cs::id<1>{i}], v2j);
+            }
+        }
+    }
+};
+
+struct GemvTPartial {
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::f32, 2> B;
+        cs::Input<DType::f32, 1> y;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 2> P;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t n = u32(in.B.shape()[1]), R = u32(in.B.shape()[0] / out.P.shape()[0]), total = (n / R) * n;
+        const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        for (std::uint32_t t = u32(it.get_global_id(0)); t < total; t += stride) {
+            const std::uint32_t c = t / n, j = t - c * n;
+            float               acc = 0.0f;
+            for (std::uint32_t k = 0; k < R; ++k) {
+                const std::uint32_t i = c * R + k;
+                acc                   = fmaf(in.B[cs::id<2>{i, j}], in.y[cs::id<1>{i}], acc);
+            }
+            out.P[cs::id<2>{c, j}] = acc;
+        }
+    }
+};
+
+template <bool Z>
+struct ColReduce {
+    static constexpr int max_work_group_size = kWg;
+    struct InputsZ {
+        cs::Input<DType::f32, 2> P;
+        cs::Input<DType::f32, 1> z;
+        cs::Input<DType::f32, 1> scale;
+    };
+    struct InputsNoZ {
+        cs::Input<DType::f32, 2> P;
+        cs::Input<DType::f32, 1> scale;
+    };
+    using Inputs = std::conditional_t<Z, InputsZ, InputsNoZ>;
+    struct Outputs {
+        cs::Output<DType::f32, 1> x;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t n = u32(in.P.shape()[1]), strips = u32(in.P.shape()[0]);
+        const float         scale  = in.scale[std::size_t{0}];
+        const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
+            float acc = 0.0f;
+            for (std::uint32_t c = 0; c < strips; ++c) acc += in.P[cs::id<2>{c, j}];
+            if constexpr (Z)
+                out.x[cs::id<1>{j}] = fmaf(scale, acc, in.z[cs::id<1>{j}]);
+            else
+                out.x[cs::id<1>{j}] = scale * acc;
+        }
+    }
+};
+
+struct Gemv {
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::f32, 2> B;
+        cs::Input<DType::f32, 1> x;
+        cs::Input<DType::f32, 1> scale;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> y;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t n     = u32(in.B.shape()[0]);
+        const float         scale = in.scale[std::size_t{0}];
+        const std::uint32_t l = u32(it.get_local_id(0)), wg = u32(it.get_local_range(0));
+        const std::uint32_t g = u32(it.get_group(0)), ng = u32(it.get_group_range(0));
+        for (std::uint32_t i = g; i < n; i += ng) {
+            float acc = 0.0f;
+            for (std::uint32_t j = l; j < n; j += wg) acc = fmaf(in.B[cs::id<2>{i, j}], in.x[cs::id<1>{j}], acc);
+            const float sum = cs::reduce_over_group(it.get_group(), acc, cs::plus<float>());
+            if (l == 0) out.y[cs::id<1>{i}] = scale * sum;
+        }
+    }
+};
+
+// =============================================================================
+// Storage of A: 16-byte vector loads, widening to f32
+// =============================================================================
+
+template <DType S>
+struct Storage;
+
+template <>
+struct Storage<DType::f32> {
+    using T                    = float;
+    static constexpr int kVec  = 4;
+    using Raw                  = float __attribute__((ext_vector_type(4)));
+    __device__ static void widen(const Raw& r, float s, float (&a)[kVec]) {
+        (void)s;
+        a[0] = r[0];
+        a[1] = r[1];
+        a[2] = r[2];
+        a[3] = r[3];
+    }
+    __device__ static float widen1(const T& v, float s) {
+        (void)s;
+        return v;
+    }
+};
+
+template <>
+struct Storage<DType::bf16> {
+    using T                   = cs::bf16;
+    static constexpr int kVec = 8;
+    using Raw                 = std::uint32_t __attribute__((ext_vector_type(4)));
+    __device__ static void widen(const Raw& r, float s, float (&a)[kVec]) {
+        (void)s;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a[2 * q]     = __uint_as_float(r[q] << 16);
+            a[2 * q + 1] = __uint_as_float(r[q] & 0xffff0000u);
+        }
+    }
+    __device__ static float widen1(const T& v, float s) {
+        (void)s;
+        return v.to_float();
+    }
+};
+
+template <>
+struct Storage<DType::e4m3> {
+    using T                   = cs::e4m3;
+    static constexpr int kVec = 16;
+    using Raw                 = std::uint32_t __attribute__((ext_vector_type(4)));
+    __device__ static void widen(const Raw& r, float s, float (&a)[kVec]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(r[q]), false);
+            const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(r[q]), true);
+            a[4 * q]      = lo[0] * s;
+            a[4 * q + 1]  = lo[1] * s;
+            a[4 * q + 2]  = hi[0] * s;
+            a[4 * q + 3]  = hi[1] * s;
+        }
+    }
+    __device__ static float widen1(const T& v, float s) {
+        const auto p = __builtin_amdgcn_cvt_pk_f32_fp8(static_cast<int>(v.bits), false);
+        return p[0] * s;
+    }
+};
+
+/// A streaming 16-byte load (A is read once per pass: nontemporal, so it does not evict the reused vectors).
+template <class Raw>
+__device__ inline Raw load_stream(const void* p) {
+    return __builtin_nontemporal_load(static_cast<const Raw*>(p));
+}
+
+/// N floats from an f32 vector port at element j (16-byte aligned when N % 4 == 0 and j % 4 == 0).
+template <int N>
+__device__ inline void load_f32(const float* p, float (&v)[N]) {
+    static_assert(N % 4 == 0, "cospan: f32 vectors load 4 at a time");
+    using F4 = float __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int q = 0; q < N / 4; ++q) {
+        const F4 t   = *reinterpret_cast<const F4*>(p + 4 * q);
+        v[4 * q]     = t[0];
+        v[4 * q + 1] = t[1];
+        v[4 * q + 2] = t[2];
+        v[4 * q + 3] = t[3];
+    }
+}
+
+// =============================================================================
+// Pass 1: P[c][j] = sum_{k<R} Â[cR + k][j] y[cR + k]
+// =============================================================================
+
+template <DType S>
+struct AhatTPartial {
+    static constexpr int max_work_group_size = kWg;
+    using St                                 = Storage<S>;
+    static constexpr int kVec                = St::kVec;
+    /// columns of one work-group
+    static constexpr int kCols = kWg * kVec;
+
+    struct Inputs {
+        cs::Input<S, 2>          A;
+        cs::Input<DType::f32, 1> row_scale;  ///< e4m3 storage only (extent 0 otherwise)
+        cs::Input<DType::f32, 1> u1, v1, u2, v2, y;
+        cs::Input<DType::u32, 1> R;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 2> P;
+    };
+
+    /// group (c, cb): strip c of R rows, column block cb of kCols columns
+    __device__ static void compute_forward(cs::GroupItem<2> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
+        const std::size_t   lda  = in.A.stride()[0];
+        const std::uint32_t R    = in.R[std::size_t{0}];
+        const std::uint32_t c = u32(it.get_group(0)), cb = u32(it.get_group(1));
+        const std::uint32_t j0 = (cb * kWg + u32(it.get_local_id(1))) * kVec;
+        if (j0 >= n) return;
+        const std::uint32_t i0 = c * R, i1 = (i0 + R < rows) ? i0 + R : rows;
+        const auto*         A  = cs::raw::data(in.A);
+        const float*        u1 = cs::raw::data(in.u1);
+        const float*        u2 = cs::raw::data(in.u2);
+        const float*        y  = cs::raw::data(in.y);
+        const float*        sc = cs::raw::data(in.row_scale);
+        float               acc[kVec];
+#pragma unroll
+        for (int v = 0; v < kVec; ++v) acc[v] = 0.0f;
+
+        if (j0 + kVec <= n) {
+            float v1[kVec], v2[kVec];
+            load_f32<kVec>(cs::raw::data(in.v1) + j0, v1);
+            load_f32<kVec>(cs::raw::data(in.v2) + j0, v2);
+            const typename St::T* a = A + static_cast<std::size_t>(i0) * lda + j0;
+#pragma unroll 8
+            for (std::uint32_t i = i0; i < i1; ++i, a += lda) {
+                const typename St::Raw r = load_stream<typename St::Raw>(a);
+                float                  av[kVec];
+                St::widen(r, S == DType::e4m3 ? sc[i] : 1.0f, av);
+                const float yi = y[i], a1 = u1[i], a2 = u2[i];
+#pragma unroll
+                for (int v = 0; v < kVec; ++v) acc[v] = fmaf(rank2(av[v], a1, v1[v], a2, v2[v]), yi, acc[v]);
+            }
+            float* p = &out.P[cs::id<2>{c, j0}];
+#pragma unroll
+            for (int q = 0; q < kVec / 4; ++q) {
+                using F4                        = float __attribute__((ext_vector_type(4)));
+                reinterpret_cast<F4*>(p)[q]     = F4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+            }
+        } else {
+            // the ragged tail of a row: element by element, the same order
+            const std::uint32_t m = n - j0;
+            for (std::uint32_t i = i0; i < i1; ++i) {
+                const float s  = S == DType::e4m3 ? sc[i] : 1.0f;
+                const float yi = y[i], a1 = u1[i], a2 = u2[i];
+                for (std::uint32_t v = 0; v < m; ++v) {
+                    const float av = St::widen1(A[static_cast<std::size_t>(i) * lda + j0 + v], s);
+                    acc[v]         = fmaf(rank2(av, a1, in.v1[cs::id<1>{j0 + v}], a2, in.v2[cs::id<1>{j0 + v}]), yi, acc[v]);
+                }
+            }
+            for (std::uint32_t v = 0; v < m; ++v) out.P[cs::id<2>{c, j0 + v}] = acc[v];
+        }
+    }
+};
+
+// =============================================================================
+// Pass 2: w[i] = alpha * sum_j Â[i][j] x[j], RT rows per work-group
+// =============================================================================
+
+template <DType S, int RT>
+struct AhatGemv {
+    static constexpr int max_work_group_size = kWg;
+    using St                                 = Storage<S>;
+    static constexpr int kVec                = St::kVec;
+    static constexpr int kRows               = RT;
+    static constexpr bool kCompensated       = S != DType::f32;
+
+    struct Inputs {
+        cs::Input<S, 2>          A;
+        cs::Input<DType::f32, 1> row_scale;
+        cs::Input<DType::f32, 1> u1, v1, u2, v2, x;
+        cs::Input<DType::f32, 1> scale;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> w;
+    };
+
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
+        const std::size_t   lda  = in.A.stride()[0];
+        const std::uint32_t r0   = u32(it.get_group(0)) * RT;
+        if (r0 >= rows) return;  // uniform over the work-group
+        const std::uint32_t nr  = rows - r0 < static_cast<std::uint32_t>(RT) ? rows - r0 : static_cast<std::uint32_t>(RT);
+        const std::uint32_t tid = u32(it.get_local_id(0));
+        const auto*         A   = cs::raw::data(in.A);
+        const float*        sc  = cs::raw::data(in.row_scale);
+        const float*        xv  = cs::raw::data(in.x);
+        const float*        v1p = cs::raw::data(in.v1);
+        const float*        v2p = cs::raw::data(in.v2);
+
+        float a1[RT], a2[RT], s[RT], acc[RT], comp[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const std::uint32_t i = r0 + (static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u);
+            a1[r]                 = in.u1[cs::id<1>{i}];
+            a2[r]                 = in.u2[cs::id<1>{i}];
+            s[r]                  = S == DType::e4m3 ? sc[i] : 1.0f;
+            acc[r]                = 0.0f;
+            comp[r]               = 0.0f;
+        }
+
+        const std::uint32_t step = kWg * kVec;
+        std::uint32_t       j0   = tid * kVec;
+        for (; j0 + kVec <= n; j0 += step) {
+            float v1[kVec], v2[kVec], x[kVec];
+            load_f32<kVec>(v1p + j0, v1);
+            load_f32<kVec>(v2p + j0, v2);
+            load_f32<kVec>(xv + j0, x);
+            typename St::Raw raw[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                raw[r]                 = load_stream<typename St::Raw>(A + static_cast<std::size_t>(r0 + rr) * lda + j0);
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                float av[kVec];
+                St::widen(raw[r], s[r], av);
+                if constexpr (kCompensated) {
+                    float part = 0.0f;
+#pragma unroll
+                    for (int v = 0; v < kVec; ++v) part = fmaf(rank2(av[v], a1[r], v1[v], a2[r], v2[v]), x[v], part);
+                    // Neumaier: acc + comp is the running sum of the chunk partials
+                    const float t = acc[r] + part;
+                    comp[r] += fabsf(acc[r]) >= fabsf(part) ? (acc[r] - t) + part : (part - t) + acc[r];
+                    acc[r] = t;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < kVec; ++v) acc[r] = fmaf(rank2(av[v], a1[r], v1[v], a2[r], v2[v]), x[v], acc[r]);
+                }
+            }
+        }
+        if (j0 < n) {  // the ragged tail of the rows (this work-item's last, partial chunk)
+            for (int r = 0; r < RT; ++r) {
+                const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                float               part = 0.0f;
+                for (std::uint32_t j = j0; j < n; ++j) {
+                    const float av = St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], s[r]);
+                    part           = fmaf(rank2(av, a1[r], v1p[j], a2[r], v2p[j]), xv[j], part);
+                }
+                if constexpr (kCompensated) {
+                    const float t = acc[r] + part;
+                    comp[r] += fabsf(acc[r]) >= fabsf(part) ? (acc[r] - t) + part : (part - t) + acc[r];
+                    acc[r] = t;
+                } else {
+                    acc[r] += part;
+                }
+            }
+        }
+
+        // per row: wave butterfly, then the pairwise sum of the four wave results
+        __shared__ float wave_sums[kWg / cs::wave::kSize][RT];
+        const int        wv = static_cast<int>(tid) / cs::wave::kSize;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            float v = kCompensated ? acc[r] + comp[r] : acc[r];
+            v       = cs::wave::reduce(v, cs::plus<float>{});
+            if (cs::wave::lane_id() == 0) wave_sums[wv][r] = v;
+        }
+        __syncthreads();
+        if (tid < nr) {
+            const float alpha = in.scale[std::size_t{0}];
+            const float sum   = (wave_sums[0][tid] + wave_sums[1][tid]) + (wave_sums[2][tid] + wave_sums[3][tid]);
+            out.w[cs::id<1>{r0 + tid}] = alpha * sum;
+        }
+    }
+};
+
+}  // namespace gemver

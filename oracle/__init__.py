@@ -1,0 +1,144 @@
+"""ctypes bindings of the CPU oracle (oracle/oracle.h). TEST INFRASTRUCTURE ONLY: imported by tests/, by
+__graft_entry__.smoke() as the checker, and by bench.py's cpu_baseline leg; never by the product."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_P = C.c_void_p
+_S = C.c_uint64
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is not built: make -C oracle")
+    return C.CDLL(path)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = _load(os.path.join(HERE, "liboracle.so"))
+        sig = {
+            "or_mix32": (C.c_uint32, [C.c_uint32]),
+            "or_hash_unit": (C.c_float, [C.c_uint32, _S]),
+            "or_fill_hash_f32": (None, [C.c_uint32, _P, _S, _S, _S]),
+            "or_array_seed": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "or_relative_l2_f32": (C.c_double, [_P, _P, _S]),
+            "or_relative_l2_f64": (C.c_double, [_P, _P, _S]),
+            "or_relative_linf_f32": (C.c_double, [_P, _P, _S]),
+            "or_f32_to_bf16": (C.c_uint16, [C.c_float]),
+            "or_bf16_to_f32": (C.c_float, [C.c_uint16]),
+            "or_f32_to_e4m3": (C.c_uint8, [C.c_float]),
+            "or_e4m3_to_f32": (C.c_float, [C.c_uint8]),
+            "or_quantize_bf16": (None, [_P, _P, _S, _S, _S, _S]),
+            "or_quantize_e4m3_rows": (None, [_P, _P, _P, _S, _S, _S, _S]),
+            "or_gemver_f64": (None, [_S, C.c_int, _P, _S, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double, _P, _P]),
+            "or_rank2_update_f32": (None, [_S, _S, _P, _P, _P, _P, _P, _P]),
+            "or_gemvt_partial_f32": (None, [_S, _S, _P, _P, _P]),
+            "or_col_reduce_f32": (None, [_S, _S, _P, _P, C.c_float, _P]),
+            "or_gemv_f32": (None, [_S, _P, _P, C.c_float, _P]),
+            "or_gemver_twopass_f32": (None, [_S, _S, _P, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+            "or_exclusive_scan_u32": (None, [_P, _P, _S]),
+            "or_exclusive_scan_u64": (None, [_P, _P, _S]),
+            "or_build_column": (C.c_uint32, [C.c_int, _P, _S, _S, C.c_int, _P, _P, _P]),
+            "or_ragged_sizes": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+            "or_ns_fields": (None, [C.c_uint32, C.c_int, C.c_int, _P, _P, _P]),
+            "or_ns_residual_f64": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
+            "or_ns_compare": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "or_threads": (C.c_int, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference's own chain_data.hpp behind oracle/ref_chain.cpp, or None where it was not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libchain_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        R.ref_hash_unit.restype = C.c_float
+        R.ref_hash_unit.argtypes = [C.c_uint32, _S]
+        R.ref_make_vector.argtypes = [C.c_uint32, _S, _P]
+        R.ref_make_matrix_A0.argtypes = [_S, _P]
+        R.ref_chain.argtypes = [_S] + [_P] * 13
+        R.ref_relative_l2.restype = C.c_double
+        R.ref_relative_l2.argtypes = [_P, _P, _S]
+        _ref = R
+    return _ref
+
+
+def ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---- convenience wrappers -----------------------------------------------------------------------------------------
+SEEDS = {"A": 1, "u1": 2, "v1": 3, "u2": 4, "v2": 5, "y": 6, "z": 7}
+
+
+def hash_matrix(seed, rows, cols):
+    a = np.empty((rows, cols), np.float32)
+    lib().or_fill_hash_f32(seed, ptr(a), rows, cols, cols)
+    return a
+
+
+def gemver_inputs(n, base_seed=0):
+    """A and the six vectors, as the models generate them on the device."""
+    out = {"A": hash_matrix(base_seed * 16 + SEEDS["A"], n, n)}
+    for k in ("u1", "v1", "u2", "v2", "y", "z"):
+        out[k] = hash_matrix(base_seed * 16 + SEEDS[k], 1, n)[0]
+    return out
+
+
+def gemver_f64(inp, alpha, beta, store=0, A=None, row_scale=None):
+    n = inp["u1"].shape[0]
+    A = inp["A"] if A is None else A
+    x = np.empty(n)
+    w = np.empty(n)
+    lib().or_gemver_f64(n, store, ptr(A), n, ptr(row_scale) if row_scale is not None else None,
+                        *[ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")], alpha, beta, ptr(x), ptr(w))
+    return x, w
+
+
+def rel_l2(got, want):
+    got = np.ascontiguousarray(got, np.float32)
+    want = np.ascontiguousarray(want, np.float64)
+    return lib().or_relative_l2_f32(ptr(got), ptr(want), got.size)
+
+
+def rel_linf(got, want):
+    got = np.ascontiguousarray(got, np.float32)
+    want = np.ascontiguousarray(want, np.float64)
+    return lib().or_relative_linf_f32(ptr(got), ptr(want), got.size)
+
+
+def exclusive_scan_u32(v):
+    v = np.ascontiguousarray(v, np.uint32)
+    out = np.empty(v.size + 1, np.uint32)
+    lib().or_exclusive_scan_u32(ptr(v), ptr(out), v.size)
+    return out
+
+
+def build_column(src, offsets_mode=False, k=1):
+    """(extents, offs, total, flag) of shape_hook.hpp:215-348 restated."""
+    src = np.ascontiguousarray(src)
+    dt = {np.dtype(np.int32): 0, np.dtype(np.uint32): 1, np.dtype(np.int64): 2, np.dtype(np.uint64): 3}[src.dtype]
+    count = src.shape[0]
+    n = count - 1 if offsets_mode else count
+    ext = np.zeros(max(n, 0), np.uint32)
+    offs = np.zeros(max(n + 1, 1), np.uint32)
+    total = C.c_uint32(0)
+    flag = lib().or_build_column(dt, ptr(src), count, k, 1 if offsets_mode else 0, ptr(ext), ptr(offs), C.byref(total))
+    return ext, offs, total.value, flag

@@ -1,0 +1,566 @@
+/*
+ * oracle/oracle.c -- the CPU oracle of the fused-run path (TEST INFRASTRUCTURE ONLY; see oracle.h).
+ *
+ * Built with -ffp-contract=off: every fused multiply-add here is an explicit fmaf()/fma(), so the f32 restatements
+ * fix one evaluation order that the device kernels reproduce bit for bit where the summation order is the same.
+ *
+ * Pinning (how each part is checked beyond restating it):
+ *   hash / generators     against the reference's own chain_data.hpp, compiled from its sources (oracle/_ref)
+ *   or_gemver_f64         against chain_data.hpp's reference_chain (x1, w1 at alpha = beta = 1/sqrt(n)) through
+ *                         oracle/_ref; closed forms (u = v = 0 gives the textbook GEMV; y = 0 gives x = z)
+ *   f32 components        against or_gemver_f64 (tolerance) and brute force at tiny n; rank-2 update bitwise
+ *                         against a plain fmaf loop
+ *   scans / columns       std::exclusive_scan known answers of tests/test_device_scan.cpp:57-87 and
+ *                         tests/test_shapes.cpp:656-682 (golden fixtures under tests/golden)
+ *   bf16 / e4m3           exhaustive round trips over every code, round-to-nearest-even tie cases
+ *   NS residual           uniform fields give exactly 0; a single Fourier mode matches the analytic derivatives to
+ *                         O(h^2)
+ *   or_ragged_sizes       bounds and the log-uniform spread only: parity unpinned (no reference counterpart)
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int or_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---- hash: examples/gemver-chain/common/chain_data.hpp:34-49 -------------------------------------------------- */
+uint32_t or_mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+float or_hash_unit(uint32_t seed, uint64_t index) {
+    const uint32_t lo = (uint32_t)index, hi = (uint32_t)(index >> 32);
+    const uint32_t h  = or_mix32(or_mix32(seed * 0x9e3779b9u + 0x85ebca6bu) ^ or_mix32(lo + or_mix32(hi + seed)));
+    return (float)(h >> 8) * (1.0f / 8388608.0f) - 1.0f;
+}
+
+uint32_t or_array_seed(uint32_t base_seed, uint32_t id) { return base_seed * 16u + id; }
+
+/* chain_data.hpp:52-63 (make_vector / make_matrix_A0): the flat index is i * cols + j whatever the padding */
+void or_fill_hash_f32(uint32_t seed, float* out, uint64_t rows, uint64_t cols, uint64_t ld) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)rows; ++i)
+        for (uint64_t j = 0; j < cols; ++j) out[(uint64_t)i * ld + j] = or_hash_unit(seed, (uint64_t)i * cols + j);
+}
+
+/* ---- comparison: chain_data.hpp:147-161 -------------------------------------------------------------------------- */
+static double rel_l2_core(const double* num_den) {
+    const double num = num_den[0], den = num_den[1];
+    if (isnan(num)) return INFINITY;
+    if (den == 0.0) return num == 0.0 ? 0.0 : INFINITY;
+    return sqrt(num / den);
+}
+
+double or_relative_l2_f32(const float* got, const double* want, uint64_t n) {
+    double num = 0.0, den = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double d = (double)got[i] - want[i];
+        num += d * d;
+        den += want[i] * want[i];
+    }
+    const double nd[2] = {num, den};
+    return rel_l2_core(nd);
+}
+
+double or_relative_l2_f64(const double* got, const double* want, uint64_t n) {
+    double num = 0.0, den = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double d = got[i] - want[i];
+        num += d * d;
+        den += want[i] * want[i];
+    }
+    const double nd[2] = {num, den};
+    return rel_l2_core(nd);
+}
+
+double or_relative_linf_f32(const float* got, const double* want, uint64_t n) {
+    double md = 0.0, mw = 0.0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const double d = fabs((double)got[i] - want[i]);
+        if (isnan(d)) return INFINITY;
+        if (d > md) md = d;
+        if (fabs(want[i]) > mw) mw = fabs(want[i]);
+    }
+    if (mw == 0.0) return md == 0.0 ? 0.0 : INFINITY;
+    return md / mw;
+}
+
+/* ---- storage formats --------------------------------------------------------------------------------------------- */
+static uint32_t f2u(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+static float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+uint16_t or_f32_to_bf16(float f) {
+    const uint32_t u = f2u(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u); /* NaN: quiet, keep the sign */
+    const uint32_t rounded = u + 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(rounded >> 16);
+}
+
+float or_bf16_to_f32(uint16_t b) { return u2f((uint32_t)b << 16); }
+
+/* OCP E4M3: bias 7, no infinities, 0x7f / 0xff are NaN, largest finite 448 = 1.75 * 2^8, smallest subnormal 2^-9. */
+float or_e4m3_to_f32(uint8_t b) {
+    const uint32_t s = b >> 7, e = (b >> 3) & 0xfu, m = b & 7u;
+    float          v;
+    if (e == 0xfu && m == 7u) return u2f(s ? 0xffc00000u : 0x7fc00000u);
+    if (e == 0)
+        v = ldexpf((float)m, -9);
+    else
+        v = ldexpf(1.0f + (float)m / 8.0f, (int)e - 7);
+    return s ? -v : v;
+}
+
+uint8_t or_f32_to_e4m3(float f) {
+    const uint32_t u    = f2u(f);
+    const uint8_t  sign = (uint8_t)((u >> 31) << 7);
+    if (isnan(f)) return (uint8_t)(sign | 0x7f);
+    const float a = fabsf(f);
+    if (a >= 448.0f) return (uint8_t)(sign | 0x7e); /* saturate */
+    /* candidates: the value in units of the spacing of its binade, rounded to nearest even */
+    int   ex;
+    (void)frexpf(a, &ex); /* a = fr * 2^ex, fr in [0.5, 1) */
+    int e_unb = ex - 1;   /* a in [2^e_unb, 2^(e_unb+1)) */
+    if (a == 0.0f) return sign;
+    if (e_unb < -6) e_unb = -6; /* subnormal range: spacing 2^-9 */
+    const float spacing = ldexpf(1.0f, e_unb - 3);
+    const float q       = a / spacing; /* exact: a power-of-two division */
+    float       r       = nearbyintf(q); /* round to nearest even (default rounding mode) */
+    /* r * spacing is the rounded magnitude; re-encode */
+    const float mag = r * spacing;
+    if (mag >= 448.0f) return (uint8_t)(sign | 0x7e);
+    if (mag == 0.0f) return sign;
+    int ex2;
+    (void)frexpf(mag, &ex2);
+    int e2 = ex2 - 1;
+    if (e2 < -6) { /* subnormal */
+        const uint32_t m = (uint32_t)(mag / ldexpf(1.0f, -9));
+        return (uint8_t)(sign | m);
+    }
+    const uint32_t m = (uint32_t)((mag / ldexpf(1.0f, e2) - 1.0f) * 8.0f);
+    return (uint8_t)(sign | (uint8_t)((e2 + 7) << 3) | (uint8_t)m);
+}
+
+void or_quantize_bf16(const float* in, uint16_t* out, uint64_t rows, uint64_t cols, uint64_t ld_in, uint64_t ld_out) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)rows; ++i)
+        for (uint64_t j = 0; j < cols; ++j) out[(uint64_t)i * ld_out + j] = or_f32_to_bf16(in[(uint64_t)i * ld_in + j]);
+}
+
+void or_quantize_e4m3_rows(const float* in, uint8_t* out, float* scale, uint64_t rows, uint64_t cols, uint64_t ld_in,
+                           uint64_t ld_out) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)rows; ++i) {
+        const float* a    = in + (uint64_t)i * ld_in;
+        float        amax = 0.0f;
+        for (uint64_t j = 0; j < cols; ++j)
+            if (fabsf(a[j]) > amax) amax = fabsf(a[j]);
+        const float s = amax > 0.0f ? amax / 448.0f : 1.0f;
+        scale[i]      = s;
+        for (uint64_t j = 0; j < cols; ++j) out[(uint64_t)i * ld_out + j] = or_f32_to_e4m3(a[j] / s);
+    }
+}
+
+/* ---- GEMVER, FP64: chain_data.hpp:93-117 ------------------------------------------------------------------------- */
+static inline double stored(int store, const void* A, const float* row_scale, uint64_t i, uint64_t j, uint64_t lda) {
+    switch (store) {
+        case 1: return (double)or_bf16_to_f32(((const uint16_t*)A)[i * lda + j]);
+        case 2: return (double)or_e4m3_to_f32(((const uint8_t*)A)[i * lda + j]) * (double)row_scale[i];
+        default: return (double)((const float*)A)[i * lda + j];
+    }
+}
+
+void or_gemver_f64(uint64_t n, int store, const void* A, uint64_t lda, const float* row_scale, const float* u1,
+                   const float* v1, const float* u2, const float* v2, const float* y, const float* z, double alpha,
+                   double beta, double* x, double* w) {
+    /* B(i, j) = A(i, j) + (u1_i v1_j + u2_i v2_j), chain_data.hpp:99-102; x = z + beta * B^T y (i ascending per j);
+     * w = alpha * B x (j ascending per i), chain_data.hpp:104-117. Columns are split over threads for x: each column's
+     * sum keeps the reference's row order. */
+    const long long JB = 256;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long jb = 0; jb < (long long)n; jb += JB) {
+        double         acc[256];
+        const uint64_t je = (uint64_t)jb + JB < n ? (uint64_t)jb + JB : n;
+        for (uint64_t j = (uint64_t)jb; j < je; ++j) acc[j - jb] = 0.0;
+        for (uint64_t i = 0; i < n; ++i) {
+            const double yi = y[i], a1 = u1[i], a2 = u2[i];
+            for (uint64_t j = (uint64_t)jb; j < je; ++j)
+                acc[j - jb] += (stored(store, A, row_scale, i, j, lda) + (a1 * v1[j] + a2 * v2[j])) * yi;
+        }
+        for (uint64_t j = (uint64_t)jb; j < je; ++j) x[j] = (double)z[j] + beta * acc[j - jb];
+    }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        const double a1 = u1[i], a2 = u2[i];
+        double       sum = 0.0;
+        for (uint64_t j = 0; j < n; ++j) sum += (stored(store, A, row_scale, (uint64_t)i, j, lda) + (a1 * v1[j] + a2 * v2[j])) * x[j];
+        w[i] = alpha * sum;
+    }
+}
+
+/* ---- GEMVER, f32 restatement of the components ---------------------------------------------------------------------
+ * The reference's bodies leave floating-point contraction to its compiler (components.hpp:93-94,150,200); this build
+ * fixes it: b = fma(u2_i, v2_j, fma(u1_i, v1_j, a_ij)), acc = fma(b, y_i, acc), x = fma(scale, acc, z). */
+static inline float rank2(float a, float u1i, float v1j, float u2i, float v2j) { return fmaf(u2i, v2j, fmaf(u1i, v1j, a)); }
+
+/* components.hpp:85-97: strips of R rows, B[cR + k][j] */
+void or_rank2_update_f32(uint64_t n, uint64_t R, const float* A, const float* u1, const float* v1, const float* u2,
+                         const float* v2, float* B) {
+    const uint64_t rows = (n / R) * R;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)rows; ++i)
+        for (uint64_t j = 0; j < n; ++j) B[(uint64_t)i * n + j] = rank2(A[(uint64_t)i * n + j], u1[i], v1[j], u2[i], v2[j]);
+}
+
+/* components.hpp:142-155: P[c][j] = sum_{k<R} B[cR + k][j] * y[cR + k], k ascending */
+void or_gemvt_partial_f32(uint64_t n, uint64_t R, const float* B, const float* y, float* P) {
+    const uint64_t strips = n / R;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)strips; ++c) {
+        float* p = P + (uint64_t)c * n;
+        for (uint64_t j = 0; j < n; ++j) p[j] = 0.0f;
+        for (uint64_t k = 0; k < R; ++k) {
+            const uint64_t i  = (uint64_t)c * R + k;
+            const float    yi = y[i];
+            for (uint64_t j = 0; j < n; ++j) p[j] = fmaf(B[i * n + j], yi, p[j]);
+        }
+    }
+}
+
+/* components.hpp:192-204: x[j] = z[j] + scale * sum_c P[c][j], c ascending (plain adds) */
+void or_col_reduce_f32(uint64_t n, uint64_t strips, const float* P, const float* z, float scale, float* x) {
+#pragma omp parallel for schedule(static)
+    for (long long j = 0; j < (long long)n; ++j) {
+        float acc = 0.0f;
+        for (uint64_t c = 0; c < strips; ++c) acc += P[c * n + (uint64_t)j];
+        x[j] = z ? fmaf(scale, acc, z[j]) : scale * acc;
+    }
+}
+
+/* The group reduction of components.hpp:247 has an implementation-defined order; this build's is a balanced pairwise
+ * tree over the 256 work-items in local-id order (what a wave64 xor butterfly followed by a pairwise sum of the four
+ * waves computes). */
+static float tree_sum(const float* p, int m) {
+    if (m == 1) return p[0];
+    const int h = m / 2;
+    return tree_sum(p, h) + tree_sum(p + h, h);
+}
+
+/* components.hpp:237-250: work-item l sums j = l, l + 256, ... (fma, ascending), then the group reduction */
+void or_gemv_f32(uint64_t n, const float* B, const float* x, float scale, float* y) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        float lane[256];
+        for (int l = 0; l < 256; ++l) {
+            float acc = 0.0f;
+            for (uint64_t j = (uint64_t)l; j < n; j += 256) acc = fmaf(B[(uint64_t)i * n + j], x[j], acc);
+            lane[l] = acc;
+        }
+        y[i] = scale * tree_sum(lane, 256);
+    }
+}
+
+/* The fused two-pass form on the CPU: pass 1 = Rank2Update + GemvTPartial with B kept in registers (same order, so the
+ * strip partials and x are bitwise the components'); pass 2 = w_i = alpha * sum_j b_ij x_j with 16 interleaved
+ * accumulators (j mod 16), combined pairwise. */
+void or_gemver_twopass_f32(uint64_t n, uint64_t R, const float* A, const float* u1, const float* v1, const float* u2,
+                           const float* v2, const float* y, const float* z, float alpha, float beta, float* P, float* x,
+                           float* w) {
+    const uint64_t strips = n / R;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)strips; ++c) {
+        float* p = P + (uint64_t)c * n;
+        for (uint64_t j = 0; j < n; ++j) p[j] = 0.0f;
+        for (uint64_t k = 0; k < R; ++k) {
+            const uint64_t i  = (uint64_t)c * R + k;
+            const float    yi = y[i], a1 = u1[i], a2 = u2[i];
+            const float*   a  = A + i * n;
+            for (uint64_t j = 0; j < n; ++j) p[j] = fmaf(rank2(a[j], a1, v1[j], a2, v2[j]), yi, p[j]);
+        }
+    }
+    or_col_reduce_f32(n, strips, P, z, beta, x);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        float        acc[16] = {0};
+        const float  a1 = u1[i], a2 = u2[i];
+        const float* a  = A + (uint64_t)i * n;
+        uint64_t     j  = 0;
+        for (; j + 16 <= n; j += 16)
+            for (int l = 0; l < 16; ++l) acc[l] = fmaf(rank2(a[j + l], a1, v1[j + l], a2, v2[j + l]), x[j + l], acc[l]);
+        for (; j < n; ++j) acc[j % 16] = fmaf(rank2(a[j], a1, v1[j], a2, v2[j]), x[j], acc[j % 16]);
+        w[i] = alpha * tree_sum(acc, 16);
+    }
+}
+
+/* ---- scans: device_scan.hpp:43-48 (n + 1 outputs, the last is the total, running sums wrap) --------------------- */
+void or_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n) {
+    uint32_t s = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        out[i] = s;
+        s += in[i];
+    }
+    out[n] = s;
+}
+
+void or_exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n) {
+    uint64_t s = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        out[i] = s;
+        s += in[i];
+    }
+    out[n] = s;
+}
+
+/* ---- ragged columns: shape_hook.hpp:180-201 (flag bits, narrowing), :269-297 (counts), :303-330 (offsets) ------- */
+static int load_src(int dt, const void* src, uint64_t i, int64_t* sv, uint64_t* uv) {
+    switch (dt) {
+        case 0: *sv = ((const int32_t*)src)[i]; return 1;
+        case 1: *uv = ((const uint32_t*)src)[i]; return 0;
+        case 2: *sv = ((const int64_t*)src)[i]; return 1;
+        default: *uv = ((const uint64_t*)src)[i]; return 0;
+    }
+}
+
+uint32_t or_build_column(int src_dtype, const void* src, uint64_t count_src, uint64_t k, int offsets_mode,
+                         uint32_t* extents, uint32_t* offs, uint32_t* total) {
+    uint32_t flag = 0;
+    if (offsets_mode) {
+        if (count_src == 0) return 0xffffffffu; /* the host refuses this before any kernel (shape_hook.hpp:245-246) */
+        const uint64_t n = count_src - 1;
+        for (uint64_t i = 0; i < count_src; ++i) {
+            int64_t  sv = 0;
+            uint64_t uv = 0;
+            uint32_t x  = 0;
+            int      bad = 0;
+            if (load_src(src_dtype, src, i, &sv, &uv)) {
+                if (sv < 0)
+                    bad = 1;
+                else if ((uint64_t)sv > 0xffffffffull)
+                    flag |= 2u;
+                else
+                    x = (uint32_t)sv;
+            } else {
+                if (uv > 0xffffffffull)
+                    flag |= 2u;
+                else
+                    x = (uint32_t)uv;
+            }
+            if (i == 0 && x != 0) bad = 1;
+            offs[i] = x;
+            if (bad) flag |= 1u;
+        }
+        for (uint64_t i = 0; i < n; ++i) {
+            if (offs[i + 1] < offs[i]) {
+                extents[i] = 0;
+                flag |= 1u;
+            } else {
+                extents[i] = offs[i + 1] - offs[i];
+            }
+        }
+        *total = offs[n];
+        return flag;
+    }
+    const uint64_t n = count_src;
+    for (uint64_t i = 0; i < n; ++i) {
+        int64_t  sv = 0;
+        uint64_t uv = 0;
+        uint32_t x  = 0;
+        if (load_src(src_dtype, src, i * k, &sv, &uv)) {
+            if (sv < 0)
+                flag |= 1u;
+            else if ((uint64_t)sv > 0xffffffffull)
+                flag |= 2u;
+            else
+                x = (uint32_t)sv;
+        } else {
+            if (uv > 0xffffffffull)
+                flag |= 2u;
+            else
+                x = (uint32_t)uv;
+        }
+        extents[i] = x;
+    }
+    or_exclusive_scan_u32(extents, offs, n);
+    for (uint64_t i = 0; i < n; ++i)
+        if (offs[i + 1] < offs[i]) flag |= 2u;
+    *total = offs[n];
+    return flag;
+}
+
+/* ---- batched sizes ---------------------------------------------------------------------------------------------- */
+void or_ragged_sizes(uint32_t seed, uint32_t count, uint32_t lo, uint32_t hi, uint32_t* sizes) {
+    const double ratio = (double)hi / (double)lo;
+    for (uint32_t k = 0; k < count; ++k) {
+        const double u = ((double)or_hash_unit(seed, k) + 1.0) * 0.5; /* [0, 1) */
+        double       v = (double)lo * pow(ratio, u);
+        long long    r = llround(v);
+        if (r < (long long)lo) r = lo;
+        if (r > (long long)hi) r = hi;
+        sizes[k] = (uint32_t)r;
+    }
+}
+
+/* ---- incompressible NS ------------------------------------------------------------------------------------------
+ * Fields: u, v, p each the sum of 8 modes a_m / |k_m|^2 cos(2 pi (kx x + ky y) + phi_m), x = j / nx, y = i / ny,
+ * integer k in [-4, 4]^2 \ {0}, a_m ~ N(0, 1) (Box-Muller), phi_m ~ U[0, 2 pi), every draw from the hash (seed,
+ * field * 64 + m * 8 + t). Evaluated as a_m (cx_j cy_i - sx_j sy_i) with per-column and per-row tables. */
+#define OR_PI 3.14159265358979323846
+static void ns_modes(uint32_t seed, int f, int m, int* kx, int* ky, double* amp, double* phi) {
+    const uint64_t b = (uint64_t)f * 64u + (uint64_t)m * 8u;
+    *kx              = (int)floorf((or_hash_unit(seed, b + 0) + 1.0f) * 4.5f) - 4;
+    *ky              = (int)floorf((or_hash_unit(seed, b + 1) + 1.0f) * 4.5f) - 4;
+    if (*kx == 0 && *ky == 0) *kx = 1;
+    const double u1 = 1.0 - ((double)or_hash_unit(seed, b + 2) + 1.0) * 0.5; /* (0, 1] */
+    const double u2 = ((double)or_hash_unit(seed, b + 3) + 1.0) * 0.5;
+    const double g  = sqrt(-2.0 * log(u1)) * cos(2.0 * OR_PI * u2);
+    *amp            = g / (double)((*kx) * (*kx) + (*ky) * (*ky));
+    *phi            = 2.0 * OR_PI * (((double)or_hash_unit(seed, b + 4) + 1.0) * 0.5);
+}
+
+void or_ns_fields(uint32_t seed, int nx, int ny, float* u, float* v, float* p) {
+    float* outs[3] = {u, v, p};
+    for (int f = 0; f < 3; ++f) {
+        double* cx = (double*)malloc(sizeof(double) * 8 * (size_t)nx);
+        double* sx = (double*)malloc(sizeof(double) * 8 * (size_t)nx);
+        double* cy = (double*)malloc(sizeof(double) * 8 * (size_t)ny);
+        double* sy = (double*)malloc(sizeof(double) * 8 * (size_t)ny);
+        double  amp[8];
+        for (int m = 0; m < 8; ++m) {
+            int    kx, ky;
+            double phi;
+            ns_modes(seed, f, m, &kx, &ky, &amp[m], &phi);
+            for (int j = 0; j < nx; ++j) {
+                const double t = 2.0 * OR_PI * (double)kx * (double)j / (double)nx;
+                cx[m * nx + j] = cos(t);
+                sx[m * nx + j] = sin(t);
+            }
+            for (int i = 0; i < ny; ++i) {
+                const double t = 2.0 * OR_PI * (double)ky * (double)i / (double)ny + phi;
+                cy[m * ny + i] = cos(t);
+                sy[m * ny + i] = sin(t);
+            }
+        }
+        float* o = outs[f];
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < ny; ++i)
+            for (int j = 0; j < nx; ++j) {
+                double s = 0.0;
+                for (int m = 0; m < 8; ++m) s += amp[m] * (cx[m * nx + j] * cy[m * ny + i] - sx[m * nx + j] * sy[m * ny + i]);
+                o[(size_t)i * nx + j] = (float)s;
+            }
+        free(cx);
+        free(sx);
+        free(cy);
+        free(sy);
+    }
+}
+
+static inline double fld(int store, const void* a, size_t k) {
+    return store == 1 ? (double)or_bf16_to_f32(((const uint16_t*)a)[k]) : (double)((const float*)a)[k];
+}
+
+/* One cell of the residual in FP64: central differences, periodic wrap. (The reference's host residual pads the
+ * domain and sweeps rows, examples/ns-residual/reference.hpp:30-130; the incompressible physics is this build's.) */
+static inline void ns_cell(int nx, int ny, double inv2h, double invh2, double nu, int store, const void* u,
+                           const void* v, const void* p, int i, int j, double r[3]) {
+    const int    jm = j == 0 ? nx - 1 : j - 1, jp = j == nx - 1 ? 0 : j + 1;
+    const int    im = i == 0 ? ny - 1 : i - 1, ip = i == ny - 1 ? 0 : i + 1;
+    const size_t c = (size_t)i * nx + j, w = (size_t)i * nx + jm, e = (size_t)i * nx + jp;
+    const size_t s = (size_t)im * nx + j, nn = (size_t)ip * nx + j;
+    const double uc = fld(store, u, c), vc = fld(store, v, c);
+    const double ue = fld(store, u, e), uw = fld(store, u, w), un = fld(store, u, nn), us = fld(store, u, s);
+    const double ve = fld(store, v, e), vw = fld(store, v, w), vn = fld(store, v, nn), vs = fld(store, v, s);
+    const double pe = fld(store, p, e), pw = fld(store, p, w), pn = fld(store, p, nn), ps = fld(store, p, s);
+    const double ux = (ue - uw) * inv2h, uy = (un - us) * inv2h, vx = (ve - vw) * inv2h, vy = (vn - vs) * inv2h;
+    const double px = (pe - pw) * inv2h, py = (pn - ps) * inv2h;
+    const double lu = (ue + uw + un + us - 4.0 * uc) * invh2, lv = (ve + vw + vn + vs - 4.0 * vc) * invh2;
+    r[0] = uc * ux + vc * uy + px - nu * lu;
+    r[1] = uc * vx + vc * vy + py - nu * lv;
+    r[2] = ux + vy;
+}
+
+void or_ns_residual_f64(int nx, int ny, double h, double nu, int store, const void* u, const void* v, const void* p,
+                        double* Ru, double* Rv, double* Rc, double* sq) {
+    const double inv2h = 1.0 / (2.0 * h), invh2 = 1.0 / (h * h);
+    double       s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s0, s1, s2)
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            double r[3];
+            ns_cell(nx, ny, inv2h, invh2, nu, store, u, v, p, i, j, r);
+            const size_t c = (size_t)i * nx + j;
+            if (Ru) Ru[c] = r[0];
+            if (Rv) Rv[c] = r[1];
+            if (Rc) Rc[c] = r[2];
+            s0 += r[0] * r[0];
+            s1 += r[1] * r[1];
+            s2 += r[2] * r[2];
+        }
+    if (sq) {
+        sq[0] = s0;
+        sq[1] = s1;
+        sq[2] = s2;
+    }
+}
+
+void or_ns_compare(int nx, int ny, double h, double nu, int store, const void* u, const void* v, const void* p,
+                   const float* Ru, const float* Rv, const float* Rc, double* stats, double* sq) {
+    const double inv2h = 1.0 / (2.0 * h), invh2 = 1.0 / (h * h);
+    double       n0 = 0, n1 = 0, n2 = 0, d0 = 0, d1 = 0, d2 = 0;
+    double       m0 = 0, m1 = 0, m2 = 0, r0 = 0, r1 = 0, r2 = 0;
+#pragma omp parallel for schedule(static) reduction(+ : n0, n1, n2, d0, d1, d2) reduction(max : m0, m1, m2, r0, r1, r2)
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            double r[3];
+            ns_cell(nx, ny, inv2h, invh2, nu, store, u, v, p, i, j, r);
+            const size_t c  = (size_t)i * nx + j;
+            const double e0 = (double)Ru[c] - r[0], e1 = (double)Rv[c] - r[1], e2 = (double)Rc[c] - r[2];
+            n0 += e0 * e0;
+            n1 += e1 * e1;
+            n2 += e2 * e2;
+            d0 += r[0] * r[0];
+            d1 += r[1] * r[1];
+            d2 += r[2] * r[2];
+            if (fabs(e0) > m0) m0 = fabs(e0);
+            if (fabs(e1) > m1) m1 = fabs(e1);
+            if (fabs(e2) > m2) m2 = fabs(e2);
+            if (fabs(r[0]) > r0) r0 = fabs(r[0]);
+            if (fabs(r[1]) > r1) r1 = fabs(r[1]);
+            if (fabs(r[2]) > r2) r2 = fabs(r[2]);
+        }
+    const double nd0[2] = {n0, d0}, nd1[2] = {n1, d1}, nd2[2] = {n2, d2};
+    stats[0] = rel_l2_core(nd0);
+    stats[1] = r0 > 0 ? m0 / r0 : (m0 == 0 ? 0 : INFINITY);
+    stats[2] = rel_l2_core(nd1);
+    stats[3] = r1 > 0 ? m1 / r1 : (m1 == 0 ? 0 : INFINITY);
+    stats[4] = rel_l2_core(nd2);
+    stats[5] = r2 > 0 ? m2 / r2 : (m2 == 0 ? 0 : INFINITY);
+    if (sq) {
+        sq[0] = d0;
+        sq[1] = d1;
+        sq[2] = d2;
+    }
+}

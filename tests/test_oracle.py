@@ -1,0 +1,336 @@
+"""The oracle pinned: against the reference's own code (oracle/_ref), its known-answer tests (tests/golden), closed forms,
+exact arithmetic and invariants. CPU only."""
+import ctypes as C
+import json
+import math
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def scan_pattern(n):
+    # (i * 2654435761u + 7u) % 11u with i a std::size_t: 64-bit arithmetic (tests/test_device_scan.cpp:57-61)
+    return np.array([(i * 2654435761 + 7) % (2 ** 64) % 11 for i in range(n)], np.uint32)
+
+
+def fnv1a(a):
+    h = 1469598103934665603
+    for b in np.ascontiguousarray(a, np.uint32).view(np.uint8).tobytes():
+        h ^= b
+        h = (h * 1099511628211) % (1 << 64)
+    return h
+
+
+def kat_cases():
+    with open(os.path.join(GOLDEN, "scan_kat.json")) as f:
+        return json.load(f)["cases"]
+
+
+# ---- hash generator and FP64 GEMVER against the reference's own chain_data.hpp ------------------------------------------
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (the reference checkout is absent)")
+def test_hash_matches_reference_generator():
+    L, R = O.lib(), O.ref()
+    idx = list(range(2000)) + [2 ** 32 - 1, 2 ** 32, 2 ** 32 + 5, 2 ** 40 + 7, 2 ** 63 + 11]
+    for seed in range(9):
+        for i in idx:
+            assert L.or_hash_unit(seed, i) == R.ref_hash_unit(seed, i)
+    n = 64
+    a = np.empty((n, n), np.float32)
+    R.ref_make_matrix_A0(n, O.ptr(a))
+    assert np.array_equal(a, O.hash_matrix(1, n, n))
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (the reference checkout is absent)")
+def test_gemver_f64_matches_reference_chain():
+    n = 256
+    inp = O.gemver_inputs(n)
+    outs = [np.empty(n) for _ in range(6)]
+    O.ref().ref_chain(n, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")],
+                      *[O.ptr(o) for o in outs])
+    ab = 1.0 / math.sqrt(n)
+    x, w = O.gemver_f64(inp, ab, ab)
+    assert np.array_equal(x, outs[0])  # x1 of the reference's chain, bit for bit
+    assert np.array_equal(w, outs[1])  # w1
+
+
+def test_gemver_f64_closed_forms():
+    n = 96
+    inp = O.gemver_inputs(n, base_seed=3)
+    A = inp["A"].astype(np.float64)
+    # u = v = 0: the textbook GEMV pair
+    zero = dict(inp, u1=np.zeros(n, np.float32), u2=np.zeros(n, np.float32))
+    x, w = O.gemver_f64(zero, 1.5, 1.2)
+    xr = inp["z"] + 1.2 * (A.T @ inp["y"].astype(np.float64))
+    assert np.allclose(x, xr, rtol=1e-13, atol=1e-13)
+    assert np.allclose(w, 1.5 * (A @ xr), rtol=1e-12, atol=1e-12)
+    # the full rank-2 update against numpy in float64
+    x, w = O.gemver_f64(inp, 1.5, 1.2)
+    d = {k: inp[k].astype(np.float64) for k in ("u1", "v1", "u2", "v2")}
+    B = A + (np.outer(d["u1"], d["v1"]) + np.outer(d["u2"], d["v2"]))
+    xr = inp["z"] + 1.2 * (B.T @ inp["y"].astype(np.float64))
+    assert np.allclose(x, xr, rtol=1e-12, atol=1e-12)
+    assert np.allclose(w, 1.5 * (B @ xr), rtol=1e-11, atol=1e-11)
+    # y = 0: x = z exactly
+    x, _ = O.gemver_f64(dict(inp, y=np.zeros(n, np.float32)), 1.5, 1.2)
+    assert np.array_equal(x, inp["z"].astype(np.float64))
+
+
+def round_f32(fr):
+    """The f32 nearest to an exact rational, ties to even."""
+    f = np.float32(float(fr))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        d = abs(Fraction(float(c)) - fr)
+        key = (d, int(np.array(c, np.float32).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return best[1]
+
+
+def test_rank2_update_is_two_exact_fmas():
+    n, R = 16, 4
+    inp = O.gemver_inputs(n, base_seed=5)
+    B = np.empty((n, n), np.float32)
+    O.lib().or_rank2_update_f32(n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2")], O.ptr(B))
+    for i in range(n):
+        for j in range(n):
+            t = round_f32(Fraction(float(inp["u1"][i])) * Fraction(float(inp["v1"][j])) + Fraction(float(inp["A"][i, j])))
+            b = round_f32(Fraction(float(inp["u2"][i])) * Fraction(float(inp["v2"][j])) + Fraction(float(t)))
+            assert B[i, j] == b
+
+
+def test_f32_components_against_fp64():
+    n, R = 512, 64
+    inp = O.gemver_inputs(n)
+    L = O.lib()
+    B = np.empty((n, n), np.float32)
+    P = np.empty((n // R, n), np.float32)
+    x = np.empty(n, np.float32)
+    w = np.empty(n, np.float32)
+    L.or_rank2_update_f32(n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2")], O.ptr(B))
+    L.or_gemvt_partial_f32(n, R, O.ptr(B), O.ptr(inp["y"]), O.ptr(P))
+    L.or_col_reduce_f32(n, n // R, O.ptr(P), O.ptr(inp["z"]), 1.2, O.ptr(x))
+    L.or_gemv_f32(n, O.ptr(B), O.ptr(x), 1.5, O.ptr(w))
+    xr, wr = O.gemver_f64(inp, 1.5, 1.2)
+    assert O.rel_l2(x, xr) < 1e-6 and O.rel_l2(w, wr) < 1e-6
+    # the fused two-pass CPU form: x bitwise equal to the components' x (same order), w within tolerance
+    P2 = np.empty_like(P)
+    x2 = np.empty(n, np.float32)
+    w2 = np.empty(n, np.float32)
+    L.or_gemver_twopass_f32(n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")], 1.5, 1.2,
+                            O.ptr(P2), O.ptr(x2), O.ptr(w2))
+    assert np.array_equal(x2, x) and np.array_equal(P2, P)
+    assert O.rel_l2(w2, wr) < 1e-6
+
+
+def test_gemv_tree_order_is_pairwise():
+    # one row, B = 1, x = the lane index pattern: the pairwise tree over 256 lanes of exactly representable values
+    n = 512
+    B = np.ones((n, n), np.float32)
+    x = (np.arange(n) % 7).astype(np.float32)
+    y = np.empty(n, np.float32)
+    O.lib().or_gemv_f32(n, O.ptr(B), O.ptr(x), 1.0, O.ptr(y))
+    assert np.all(y == np.float32(x.astype(np.float64).sum()))
+    # a sum whose value depends on the order: 2^24 + 1 + 1 ... pins the tree (pairs (0,1), then ((0,1),(2,3)), ...)
+    x = np.zeros(n, np.float32)
+    x[0], x[2], x[3] = 2.0 ** 24, 1.0, 1.0  # lanes 0, 2, 3 (one element each)
+    O.lib().or_gemv_f32(n, O.ptr(B), O.ptr(x), 1.0, O.ptr(y))
+    # (2^24 + 0) + (1 + 1) = 2^24 + 2 exactly under the tree; a left-to-right sum gives 2^24 (each +1 ties to even)
+    assert y[0] == np.float32(2.0 ** 24 + 2)
+
+
+# ---- scans and ragged columns ----------------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", kat_cases(), ids=lambda c: c["name"])
+def test_scan_known_answers(case):
+    v = scan_pattern(case["n"]) if case["input"] == "pattern" else np.array(case["values"], np.uint32)
+    out = O.exclusive_scan_u32(v)
+    if "expected" in case:
+        assert out.tolist() == case["expected"]
+    else:
+        assert fnv1a(out) == int(case["expected_fnv1a"]) and int(out[-1]) == case["expected_total"]
+        for i, e in case["probes"]:
+            assert int(out[i]) == e
+
+
+def test_scan_wraps_mod_2_32():
+    v = np.array([2 ** 31, 2 ** 31, 5], np.uint32)
+    assert O.exclusive_scan_u32(v).tolist() == [0, 2 ** 31, 0, 5]
+
+
+def test_build_column_counts_mode():
+    # tests/test_shapes.cpp:656-682 of the reference: counts {2, 0, 3, 1} -> offsets {0, 2, 2, 5, 6}, total 6
+    for dt in (np.int32, np.uint32, np.int64, np.uint64):
+        ext, offs, total, flag = O.build_column(np.array([2, 0, 3, 1], dt))
+        assert ext.tolist() == [2, 0, 3, 1] and offs.tolist() == [0, 2, 2, 5, 6] and total == 6 and flag == 0
+    # a rank-2 source [N, k]: column 0
+    src = np.array([[2, 9], [0, 9], [3, 9], [1, 9]], np.int32)
+    ext, offs, total, flag = O.build_column(src, k=2)
+    assert offs.tolist() == [0, 2, 2, 5, 6] and flag == 0
+    # negative: bit 1, the extent is 0
+    ext, offs, total, flag = O.build_column(np.array([2, -1, 3], np.int32))
+    assert flag == 1 and ext.tolist() == [2, 0, 3] and total == 5
+    # past 32 bits: bit 2
+    ext, offs, total, flag = O.build_column(np.array([1, 2 ** 32, 1], np.uint64))
+    assert flag == 2 and ext.tolist() == [1, 0, 1]
+    # a wrap of the running sum: bit 2
+    ext, offs, total, flag = O.build_column(np.array([2 ** 31, 2 ** 31, 1], np.uint32))
+    assert flag == 2
+    # empty
+    ext, offs, total, flag = O.build_column(np.zeros(0, np.int32))
+    assert offs.tolist() == [0] and total == 0 and flag == 0
+
+
+def test_build_column_offsets_mode():
+    ext, offs, total, flag = O.build_column(np.array([0, 2, 2, 5, 6], np.int64), offsets_mode=True)
+    assert ext.tolist() == [2, 0, 3, 1] and offs.tolist() == [0, 2, 2, 5, 6] and total == 6 and flag == 0
+    # offs[0] != 0: bit 1
+    assert O.build_column(np.array([1, 2, 3], np.uint32), offsets_mode=True)[3] == 1
+    # a decrease: bit 1, extent 0
+    ext, offs, total, flag = O.build_column(np.array([0, 4, 3], np.int32), offsets_mode=True)
+    assert flag == 1 and ext.tolist() == [4, 0]
+    # a negative entry: bit 1
+    assert O.build_column(np.array([0, -3, 3], np.int32), offsets_mode=True)[3] & 1
+
+
+# ---- storage formats -------------------------------------------------------------------------------------------------------
+
+def test_bf16_round_trip_and_rounding():
+    L = O.lib()
+    for b in range(0, 1 << 16, 7):
+        f = L.or_bf16_to_f32(b)
+        if math.isnan(f):
+            assert math.isnan(L.or_bf16_to_f32(L.or_f32_to_bf16(f)))
+        else:
+            assert L.or_f32_to_bf16(f) == b
+    # ties to even: 1 + 2^-8 is halfway between 1 and 1 + 2^-7
+    assert L.or_f32_to_bf16(1.0 + 2.0 ** -8) == L.or_f32_to_bf16(1.0)
+    assert L.or_f32_to_bf16(1.0 + 3 * 2.0 ** -8) == L.or_f32_to_bf16(1.0 + 2.0 ** -6)
+
+
+def e4m3_spec(b):
+    """OCP E4M3 decode from the format's definition."""
+    s = -1.0 if b & 0x80 else 1.0
+    e, m = (b >> 3) & 0xF, b & 7
+    if e == 0xF and m == 7:
+        return float("nan")
+    if e == 0:
+        return s * m * 2.0 ** -9
+    return s * (1 + m / 8) * 2.0 ** (e - 7)
+
+
+def test_e4m3_every_code():
+    L = O.lib()
+    for b in range(256):
+        want = e4m3_spec(b)
+        got = L.or_e4m3_to_f32(b)
+        if math.isnan(want):
+            assert math.isnan(got)
+            continue
+        assert got == want
+        if b != 0x80:  # -0 encodes as -0 (0x80), fine either way
+            assert L.or_f32_to_e4m3(want) == b
+    assert L.or_e4m3_to_f32(0x7E) == 448.0
+    assert L.or_f32_to_e4m3(1e9) == 0x7E and L.or_f32_to_e4m3(-1e9) == 0xFE
+    # ties to even: 1 + 1/16 is halfway between 1 (m = 0) and 1 + 1/8 (m = 1) -> 1
+    assert L.or_f32_to_e4m3(1.0625) == L.or_f32_to_e4m3(1.0)
+    assert L.or_f32_to_e4m3(1.1875) == L.or_f32_to_e4m3(1.25)
+    # the smallest subnormal and half of it (ties to even: 0)
+    assert L.or_f32_to_e4m3(2.0 ** -9) == 1 and L.or_f32_to_e4m3(2.0 ** -10) == 0
+
+
+def test_e4m3_nearest_is_nearest():
+    L = O.lib()
+    codes = sorted({e4m3_spec(b) for b in range(256) if not math.isnan(e4m3_spec(b))})
+    rng = np.random.default_rng(0)
+    for v in rng.uniform(-460, 460, 2000).astype(np.float32):
+        q = L.or_e4m3_to_f32(L.or_f32_to_e4m3(float(v)))
+        d = min(abs(c - float(v)) for c in codes)
+        assert abs(q - float(v)) == d or abs(float(v)) > 448
+
+
+def test_quantize_e4m3_rows_scale():
+    a = O.hash_matrix(9, 8, 100)
+    a[3] = 0.0
+    q = np.empty((8, 100), np.uint8)
+    s = np.empty(8, np.float32)
+    O.lib().or_quantize_e4m3_rows(O.ptr(a), O.ptr(q), O.ptr(s), 8, 100, 100, 100)
+    assert s[3] == 1.0 and np.all(q[3] == 0)
+    for i in (0, 5):
+        assert s[i] == np.float32(np.abs(a[i]).max()) / np.float32(448.0)
+        assert np.abs(q[i].astype(np.int32) & 0x7F).max() == 0x7E  # the row maximum maps to 448
+
+
+# ---- NS residual ----------------------------------------------------------------------------------------------------------
+
+def test_ns_uniform_field_gives_zero():
+    nx = ny = 32
+    u = np.full(nx * ny, 0.75, np.float32)
+    v = np.full(nx * ny, -1.25, np.float32)
+    p = np.full(nx * ny, 3.0, np.float32)
+    R = [np.empty(nx * ny) for _ in range(3)]
+    sq = np.empty(3)
+    O.lib().or_ns_residual_f64(nx, ny, 1.0 / nx, 1e-3, 0, O.ptr(u), O.ptr(v), O.ptr(p), *[O.ptr(r) for r in R], O.ptr(sq))
+    assert all(np.all(r == 0.0) for r in R) and np.all(sq == 0.0)
+
+
+def test_ns_single_mode_matches_analytic_derivatives():
+    n = 256
+    h = 1.0 / n
+    nu = 1e-3
+    jj, ii = np.meshgrid(np.arange(n), np.arange(n))
+    X, Y = jj * h, ii * h
+    k = 2 * np.pi
+    u = np.cos(k * X) * np.sin(k * Y)
+    v = -np.sin(k * X) * np.cos(k * Y)
+    p = 0.25 * (np.cos(2 * k * X) + np.cos(2 * k * Y))
+    uf, vf, pf = (a.astype(np.float32).ravel() for a in (u, v, p))
+    R = [np.empty(n * n) for _ in range(3)]
+    O.lib().or_ns_residual_f64(n, n, h, nu, 0, O.ptr(uf), O.ptr(vf), O.ptr(pf), *[O.ptr(r) for r in R], None)
+    U, V, Pp = (a.astype(np.float32).astype(np.float64) for a in (u, v, p))
+    ux = -k * np.sin(k * X) * np.sin(k * Y)
+    uy = k * np.cos(k * X) * np.cos(k * Y)
+    vx = -k * np.cos(k * X) * np.cos(k * Y)
+    vy = k * np.sin(k * X) * np.sin(k * Y)
+    px = -0.5 * k * np.sin(2 * k * X)
+    py = -0.5 * k * np.sin(2 * k * Y)
+    lu, lv = -2 * k * k * u, -2 * k * k * v
+    Ru = U * ux + V * uy + px - nu * lu
+    Rv = U * vx + V * vy + py - nu * lv
+    Rc = ux + vy
+    for got, want in zip(R, (Ru, Rv, Rc)):
+        err = np.abs(got.reshape(n, n) - want).max()
+        assert err < 2e-3 * max(1.0, np.abs(want).max())  # O(h^2) truncation (k h ~ 0.025) plus f32 rounding of the inputs
+
+
+def test_ns_compare_agrees_with_residual():
+    nx, ny = 64, 48
+    u, v, p = (np.empty(nx * ny, np.float32) for _ in range(3))
+    O.lib().or_ns_fields(7, nx, ny, O.ptr(u), O.ptr(v), O.ptr(p))
+    R = [np.empty(nx * ny) for _ in range(3)]
+    sq = np.empty(3)
+    O.lib().or_ns_residual_f64(nx, ny, 1.0 / nx, 1e-3, 0, O.ptr(u), O.ptr(v), O.ptr(p), *[O.ptr(r) for r in R], O.ptr(sq))
+    Rf = [r.astype(np.float32) for r in R]
+    stats = np.empty(6)
+    sq2 = np.empty(3)
+    O.lib().or_ns_compare(nx, ny, 1.0 / nx, 1e-3, 0, O.ptr(u), O.ptr(v), O.ptr(p), *[O.ptr(r) for r in Rf], O.ptr(stats), O.ptr(sq2))
+    assert np.all(stats < 1e-7) and np.allclose(sq, sq2, rtol=1e-12)
+    assert np.allclose(sq, [np.sum(r * r) for r in R], rtol=1e-10)
+    # the fields are not trivial: every field has spread
+    assert min(u.std(), v.std(), p.std()) > 1e-3
+
+
+def test_ragged_sizes_bounds():
+    s = np.empty(512, np.uint32)
+    O.lib().or_ragged_sizes(0, 512, 64, 4096, O.ptr(s))
+    assert s.min() >= 64 and s.max() <= 4096
+    logs = np.log(s.astype(np.float64) / 64) / np.log(64)
+    assert 0.4 < logs.mean() < 0.6 and logs.min() < 0.05 and logs.max() > 0.95
