@@ -85,6 +85,7 @@ __device__ inline void finish(ScanHeader* ws, uint64_t* status, uint32_t tiles, 
         result[0] = total_from_status ? static_cast<uint32_t>(atomic_load_u64(&status[tiles - 1])) : result_total;
         result[1] = atomic_load_u32(&ws->flag);
     }
+    __syncthreads();  // the total is read before any work-item of this group zeroes the status words
     if (status)
         for (uint32_t t = static_cast<uint32_t>(tid); t < tiles; t += kThreads) atomic_store_u64(&status[t], 0);
     if (tid == 0) {

@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstring>
+#include <type_traits>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -85,6 +87,36 @@ private:
     std::size_t    words_ = 0;
 };
 
+/// One prepared launch: the kernel's host stub, its shape and a copy of its argument bytes. Submitting it is one
+/// cs_hip_launch; two runs whose records are byte-identical launch identical work (the test for graph replay).
+struct LaunchRecord {
+    const void*            kernel   = nullptr;
+    std::uint32_t          grid[3]  = {0, 0, 0};
+    std::uint32_t          block[3] = {1, 1, 1};
+    std::size_t            lds      = 0;
+    std::vector<std::byte> args;  ///< the one by-value kernel argument (GroupArgs<X...>)
+    std::string            name;
+
+    bool empty() const { return kernel == nullptr; }
+    void submit(cs_hip_stream s) const {
+        if (empty()) return;
+        void* a[] = {const_cast<std::byte*>(args.data())};
+        hip_check(cs_hip_launch(kernel, grid, block, lds, a, s), "launch of '" + name + "'");
+    }
+    /// Appends the identity of this launch to `sig`.
+    void append_signature(std::vector<std::byte>& sig) const {
+        auto put = [&](const void* p, std::size_t n) {
+            const auto* b = static_cast<const std::byte*>(p);
+            sig.insert(sig.end(), b, b + n);
+        };
+        put(&kernel, sizeof kernel);
+        put(grid, sizeof grid);
+        put(block, sizeof block);
+        put(&lds, sizeof lds);
+        put(args.data(), args.size());
+    }
+};
+
 /// Builds the geometry of a group and launches its kernel.
 template <class... X>
 class GroupLauncher {
@@ -109,6 +141,14 @@ public:
     /// Launches the group on `s`. `split`: the range_split strategy; `waits`: group_waits_enabled or a global_barrier.
     void launch(Split split, bool waits, const std::vector<SegmentPlan>& segs, const StagePlan (&stages)[N], const ArgPack<X...>& args,
                 cs_hip_stream s, const std::string& group_name) {
+        prepare(split, waits, segs, stages, args, s, group_name).submit(s);
+    }
+
+    /// The launch of the group as a record (nothing is submitted; the ticket state is allocated on `s` when first needed).
+    LaunchRecord prepare(Split split, bool waits, const std::vector<SegmentPlan>& segs, const StagePlan (&stages)[N],
+                         const ArgPack<X...>& args, cs_hip_stream s, const std::string& group_name) {
+        LaunchRecord    rec;
+        rec.name = group_name;
         GroupArgs<X...> ga{};
         ga.args                 = args;
         Geometry<N>&  geo       = ga.geo;
@@ -173,9 +213,14 @@ public:
         if (lds > 160u * 1024u)
             throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': " + std::to_string(lds) +
                                         " bytes of local memory exceed the 163840 of a CDNA4 work-group");
-        if (geo.total_groups == 0) return;  // nothing to run
+        if (geo.total_groups == 0) return rec;  // nothing to run: an empty record
 
-        // physical grid (x is physical dimension 2)
+        // physical grid (x is physical dimension 2); a hardware grid allows 65535 groups in y and z, beyond that the
+        // groups are flattened along x (the logical coordinates are the same either way)
+        if (split != Split::flatten && (maxr[0] > 65535 || maxr[1] > 65535 || nseg > 65535)) {
+            split     = Split::flatten;
+            geo.split = static_cast<std::uint32_t>(split);
+        }
         std::uint32_t grid[3] = {1, 1, 1};
         if constexpr (kIndexRange) {
             grid[0] = static_cast<std::uint32_t>((geo.total_groups + 255) / 256);
@@ -200,10 +245,17 @@ public:
 
         if (waits) geo.ticket_state = tickets_.ensure(ticket_words(nseg, geo.total_groups), s);
 
-        const std::uint32_t block[3] = {wg[2], wg[1], wg[0]};
-        void*               kargs[]  = {&ga};
         static_assert(sizeof(GroupArgs<X...>) <= 4096, "cospan: a fused group's arguments exceed the 4 KiB kernel-argument limit");
-        hip_check(cs_hip_launch(kernel(), grid, block, static_cast<std::size_t>(lds), kargs, s), "launch of '" + group_name + "'");
+        static_assert(std::is_trivially_copyable_v<GroupArgs<X...>>, "cospan: a fused group's arguments are copied as bytes");
+        rec.kernel = kernel();
+        for (int d = 0; d < 3; ++d) rec.grid[d] = grid[d];
+        rec.block[0] = wg[2];
+        rec.block[1] = wg[1];
+        rec.block[2] = wg[0];
+        rec.lds      = static_cast<std::size_t>(lds);
+        rec.args.resize(sizeof ga);
+        std::memcpy(rec.args.data(), &ga, sizeof ga);
+        return rec;
     }
 
 private:

@@ -1,0 +1,276 @@
+#pragma once
+
+// =============================================================================
+// naga/cospan/system.hpp — System: children, connect, promote, tie_inputs, shape_hook, init_input
+// =============================================================================
+//
+// The structural verbs of the reference (cospanmdao/include/naga/cospan/system.hpp:120-231), over this build's registry:
+//   add_subsystem(subsystem_desc<T>(name, args...))  a component (its bare-component leaf), a System (its setup() runs
+//                                                    now) or a fused group (cs::fuse_components)
+//   connect(src, dst)                                dst's variable merges into src's (the source wins)
+//   promote<DT, R>(child_path, name)                 an alias in this System's namespace for the child's variable
+//   tie_inputs<DT, R>({members}, name, shape)        a NEW variable every member merges into
+//   shape_hook(target, cs::dim::...)                 dimensions a run sets before the walk
+//   init_input(target, init)                         the initial value of a free input
+// Paths are relative to the System ("r2u.B", "fw.gtp.P").
+
+#include <initializer_list>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <string_view>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "naga/cospan/component.hpp"
+#include "naga/cospan/model_meta.hpp"
+#include "naga/cospan/variable.hpp"
+
+namespace naga::cospan {
+
+/// A child to add: its type, its name and its constructor arguments (reference subsystem.hpp).
+template <class T, class... A>
+struct SubsystemDesc {
+    using type = T;
+    std::string      name;
+    std::tuple<A...> args;
+};
+template <class T, class... A>
+SubsystemDesc<T, std::decay_t<A>...> subsystem_desc(std::string name, A&&... args) {
+    return SubsystemDesc<T, std::decay_t<A>...>{std::move(name), std::tuple<std::decay_t<A>...>{std::forward<A>(args)...}};
+}
+
+/// Connections inside a composed system or a fused group: {{"a.y", "b.x"}, ...}, source first.
+struct ConnectsDesc {
+    std::vector<std::pair<std::string, std::string>> pairs;
+};
+inline ConnectsDesc connects_desc(std::initializer_list<std::pair<std::string, std::string>> pairs) { return ConnectsDesc{pairs}; }
+
+/// Shape-hook dimension descriptors (reference shape_hook.hpp: cs::dim::fixed_at_run, as_is).
+namespace dim {
+struct FixedAtRun {
+    std::size_t value;
+};
+inline FixedAtRun fixed_at_run(std::size_t v) { return FixedAtRun{v}; }
+struct AsIs {};
+inline constexpr AsIs as_is{};
+}  // namespace dim
+
+class System;
+
+namespace detail {
+/// A fused group: a leaf with its own namespace of stages.
+class FusedBase : public Leaf {
+public:
+    virtual void build(ModelMeta& m, const std::string& path) = 0;
+};
+
+template <class T>
+struct is_subsystem_desc : std::false_type {};
+template <class T, class... A>
+struct is_subsystem_desc<SubsystemDesc<T, A...>> : std::true_type {};
+
+template <class T, class Tuple, std::size_t... I>
+std::unique_ptr<T> make_from(Tuple& t, std::index_sequence<I...>) {
+    return std::make_unique<T>(std::get<I>(t)...);
+}
+template <class T, class... A>
+std::unique_ptr<T> make_subsystem(std::tuple<A...>& args) {
+    return make_from<T>(args, std::index_sequence_for<A...>{});
+}
+struct SystemAccess;
+}  // namespace detail
+
+class System {
+public:
+    virtual ~System() = default;
+
+    /// USER: structure and wiring.
+    virtual void setup() {}
+
+    const std::string& path() const { return path_; }
+
+    template <class T, class... A>
+    void add_subsystem(SubsystemDesc<T, A...> desc) {
+        detail::ModelMeta& m    = meta("add_subsystem");
+        const std::string  full = detail::ModelMeta::join(path_, desc.name);
+        if constexpr (std::is_base_of_v<ComponentBase, T>) {
+            std::unique_ptr<T> c = detail::make_subsystem<T>(desc.args);
+            detail::ComponentAccess::bind(*c, full, &m);
+            c->declare();
+            auto drv  = std::make_unique<detail::SoloDriver<T>>(std::move(c));
+            drv->path = full;
+            m.leaves.push_back(drv.get());
+            leaves_.push_back(std::move(drv));
+        } else if constexpr (std::is_base_of_v<detail::FusedBase, T>) {
+            std::unique_ptr<T> g = detail::make_subsystem<T>(desc.args);
+            g->path              = full;
+            g->build(m, full);
+            m.leaves.push_back(g.get());
+            leaves_.push_back(std::move(g));
+        } else {
+            static_assert(std::is_base_of_v<System, T>, "cospan: add_subsystem takes a component, a System or a fused group");
+            std::unique_ptr<T> s = detail::make_subsystem<T>(desc.args);
+            s->path_             = full;
+            s->meta_             = &m;
+            s->setup();
+            systems_.push_back(std::move(s));
+        }
+    }
+
+    /// Feeds `dst` (an input) from `src`; the source's variable wins (reference wiring.hpp connect).
+    void connect(std::string_view src, std::string_view dst) {
+        detail::ModelMeta& m = meta("connect");
+        auto               s = m.at(detail::ModelMeta::join(path_, src), "connect");
+        auto               d = m.at(detail::ModelMeta::join(path_, dst), "connect");
+        if (d->kind != detail::IOKind::input)
+            throw std::invalid_argument("naga::cospan: connect: the destination '" + d->path + "' is not an input");
+        if (d->var->has_producer && d->var != s->var)
+            throw std::invalid_argument("naga::cospan: connect: '" + d->path + "' already has a source");
+        m.merge(s->var, d->var, "connect '" + s->path + "' -> '" + d->path + "'");
+    }
+
+    /// An alias in this namespace for the child's variable (DT and Rank must match).
+    template <DType DT, int Rank>
+    void promote(std::string_view child_path, std::string promoted_name) {
+        detail::ModelMeta& m      = meta("promote");
+        auto               member = m.at(detail::ModelMeta::join(path_, child_path), "promote");
+        if (!dtype_compatible(member->dtype, DT) || member->rank != Rank)
+            throw std::invalid_argument("naga::cospan: promote: '" + member->path + "' is " + dtype_name(member->dtype) + " of rank " +
+                                        std::to_string(member->rank));
+        auto a   = std::make_shared<detail::Instance>(*member);
+        a->path  = detail::ModelMeta::join(path_, promoted_name);
+        m.add(a);
+    }
+
+    /// A new variable every member (inputs of dtype DT and rank Rank) merges into; the alias names it.
+    template <DType DT, int Rank>
+    void tie_inputs(std::initializer_list<std::string_view> members, std::string name, ShapeSpec shape = {}) {
+        detail::ModelMeta& m = meta("tie_inputs");
+        if (members.size() < 2) throw std::invalid_argument("naga::cospan: tie_inputs needs at least two members");
+        auto v   = std::make_shared<detail::VarMeta>();
+        v->name  = detail::ModelMeta::join(path_, name);
+        v->dtype = DT;
+        v->rank  = Rank;
+        for (int d = 0; d < shape.rank; ++d)
+            if (shape.specified[d]) {
+                v->dims[d]  = shape.extents[d];
+                v->known[d] = v->fixed[d] = true;
+            }
+        auto a   = std::make_shared<detail::Instance>();
+        a->path  = v->name;
+        a->kind  = detail::IOKind::input;
+        a->dtype = DT;
+        a->rank  = Rank;
+        a->var   = v;
+        m.add(a);
+        for (std::string_view mp : members) {
+            auto inst = m.at(detail::ModelMeta::join(path_, mp), "tie_inputs");
+            if (inst->kind != detail::IOKind::input)
+                throw std::invalid_argument("naga::cospan: tie_inputs: '" + inst->path + "' is not an input");
+            m.merge(a->var, inst->var, "tie_inputs '" + v->name + "'");
+        }
+    }
+
+    /// Dimensions of `target` set at the start of every run (cs::dim::fixed_at_run(n), cs::dim::as_is).
+    template <class... D>
+    void shape_hook(std::string_view target, D... dims) {
+        detail::ModelMeta& m    = meta("shape_hook");
+        auto               inst = m.at(detail::ModelMeta::join(path_, target), "shape_hook");
+        if (static_cast<int>(sizeof...(D)) != inst->rank)
+            throw std::invalid_argument("naga::cospan: shape_hook: '" + inst->path + "' has rank " + std::to_string(inst->rank) +
+                                        " but the hook gives " + std::to_string(sizeof...(D)) + " dimensions");
+        std::vector<std::optional<std::size_t>> fixed;
+        (fixed.push_back([](auto d) -> std::optional<std::size_t> {
+             if constexpr (std::is_same_v<decltype(d), dim::FixedAtRun>)
+                 return d.value;
+             else
+                 return std::nullopt;
+         }(dims)),
+         ...);
+        // the hooked dimensions are owned by the hook
+        for (int d = 0; d < inst->rank; ++d)
+            if (fixed[d]) {
+                if (inst->var->fixed[d] && inst->var->dims[d] != *fixed[d])
+                    throw std::invalid_argument("naga::cospan: shape_hook: dimension " + std::to_string(d) + " of '" + inst->path +
+                                                "' is specified by its definition");
+                inst->var->dims[d]  = *fixed[d];
+                inst->var->known[d] = inst->var->fixed[d] = true;
+            }
+        detail::HookRecord rec;
+        rec.target = inst->path;
+        rec.inst   = inst;
+        rec.apply  = [fixed, path = inst->path](detail::ModelMeta&, detail::VarMeta& v) {
+            for (int d = 0; d < v.rank; ++d)
+                if (fixed[d]) {
+                    v.dims[d]  = *fixed[d];
+                    v.known[d] = true;
+                }
+        };
+        m.hooks.push_back(std::move(rec));
+    }
+
+    /// The initial value of a free input (a child's input or an alias of this System).
+    void init_input(std::string_view target, InitSpec init) {
+        detail::ModelMeta& m    = meta("init_input");
+        auto               inst = m.at(detail::ModelMeta::join(path_, target), "init_input");
+        if (inst->kind != detail::IOKind::input)
+            throw std::invalid_argument("naga::cospan: init_input: '" + inst->path + "' is an output");
+        inst->var->init    = std::move(init);
+        inst->var->pending = true;
+    }
+
+private:
+    friend struct detail::SystemAccess;
+    detail::ModelMeta& meta(const char* what) const {
+        if (!meta_) throw std::logic_error(std::string{"naga::cospan: "} + what + " is legal inside setup() only");
+        return *meta_;
+    }
+
+protected:
+    std::string        path_;
+    detail::ModelMeta* meta_ = nullptr;
+
+private:
+    std::vector<std::unique_ptr<detail::Leaf>> leaves_;
+    std::vector<std::unique_ptr<System>>       systems_;
+};
+
+namespace detail {
+struct SystemAccess {
+    static void bind(System& s, std::string path, ModelMeta* m) {
+        s.path_ = std::move(path);
+        s.meta_ = m;
+    }
+};
+
+/// The System of cs::system_from(name, [connects], descs...): its children in order, then its connections.
+template <class... Descs>
+class ComposedSystem : public System {
+public:
+    ComposedSystem(ConnectsDesc c, Descs... d) : c_{std::move(c)}, d_{std::move(d)...} {}
+    void setup() override {
+        std::apply([this](auto&... d) { (this->add_subsystem(d), ...); }, d_);
+        for (auto& [src, dst] : c_.pairs) this->connect(src, dst);
+    }
+
+private:
+    ConnectsDesc          c_;
+    std::tuple<Descs...>  d_;
+};
+}  // namespace detail
+
+/// A plain System of the given children with the given connections (reference fused_component.hpp:573-582).
+template <class... Descs>
+auto system_from(std::string name, ConnectsDesc c, Descs... d) {
+    return subsystem_desc<detail::ComposedSystem<Descs...>>(std::move(name), std::move(c), std::move(d)...);
+}
+template <class... Descs>
+    requires(sizeof...(Descs) >= 1 && !(std::is_same_v<std::tuple_element_t<0, std::tuple<Descs...>>, ConnectsDesc>))
+auto system_from(std::string name, Descs... d) {
+    return subsystem_desc<detail::ComposedSystem<Descs...>>(std::move(name), ConnectsDesc{}, std::move(d)...);
+}
+
+}  // namespace naga::cospan

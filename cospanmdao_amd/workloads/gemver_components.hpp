@@ -26,11 +26,9 @@
 // widened by the gfx950 conversion instruction). Vectors and all arithmetic are f32; the rank-2 term is never quantised.
 
 #include <cstdint>
+#include <memory>
 
-#include "naga/cospan/device.hpp"
-#include "naga/cospan/dtype.hpp"
-#include "naga/cospan/kernel.hpp"
-#include "naga/cospan/wrappers.hpp"
+#include "naga/cospan/cospan.hpp"
 
 namespace gemver {
 
@@ -38,6 +36,9 @@ namespace cs = naga::cospan;
 using cs::DType;
 
 inline constexpr int kWg = 256;  // the work-group size of every GEMVER kernel
+
+/// `v` rounded up to a multiple of `m`: a group-range launch range is a whole number of work-groups.
+inline std::size_t round_up(std::size_t v, std::size_t m) { return (v + m - 1) / m * m; }
 
 __host__ __device__ inline std::uint32_t u32(std::size_t v) { return static_cast<std::uint32_t>(v); }
 
@@ -47,7 +48,8 @@ __device__ inline float rank2(float a, float u1i, float v1j, float u2i, float v2
 // The reference's four components
 // =============================================================================
 
-struct Rank2Update {
+class Rank2Update : public cs::Component<Rank2Update> {
+public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
         cs::Input<DType::f32, 2> A;
@@ -69,9 +71,37 @@ struct Rank2Update {
             }
         }
     }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    explicit Rank2Update(int R) : R_{R} {}
+    void declare() override {
+        declare_input(inputs.A, "A", cs::unspecified_shape<2>());
+        declare_input(inputs.u1, "u1", cs::unspecified_shape<1>());
+        declare_input(inputs.v1, "v1", cs::unspecified_shape<1>());
+        declare_input(inputs.u2, "u2", cs::unspecified_shape<1>());
+        declare_input(inputs.v2, "v2", cs::unspecified_shape<1>());
+        declare_input(inputs.R, "R", 1);
+        init_input("R", cs::init::fill(u32(static_cast<std::size_t>(R_))));
+        declare_output(outputs.B, "B", cs::unspecified_shape<2>());
+    }
+    void prepare_forward() override {
+        const cs::range<2> a = shape_of(inputs.A);
+        set_shape(outputs.B, a[0], a[1]);
+    }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t n    = inputs.A.shape()[1];
+        const std::size_t work = (n / static_cast<std::size_t>(R_)) * n;
+        return create_launch_config(cs::range_desc(cs::range<1>{round_up(work, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+
+private:
+    int R_;
 };
 
-struct GemvTPartial {
+class GemvTPartial : public cs::Component<GemvTPartial> {
+public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
         cs::Input<DType::f32, 2> B;
@@ -93,10 +123,24 @@ struct GemvTPartial {
             out.P[cs::id<2>{c, j}] = acc;
         }
     }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override {
+        declare_input(inputs.B, "B", cs::unspecified_shape<2>());
+        declare_input(inputs.y, "y", cs::unspecified_shape<1>());
+        declare_output(outputs.P, "P", cs::unspecified_shape<2>());  // hooked by the owning System: (n / R, n)
+    }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t range = outputs.P.shape()[0] * outputs.P.shape()[1];
+        return create_launch_config(cs::range_desc(cs::range<1>{round_up(range, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
+    }
 };
 
 template <bool Z>
-struct ColReduce {
+class ColReduce : public cs::Component<ColReduce<Z>> {
+public:
     static constexpr int max_work_group_size = kWg;
     struct InputsZ {
         cs::Input<DType::f32, 2> P;
@@ -117,6 +161,8 @@ struct ColReduce {
         const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
         for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
             float acc = 0.0f;
+            // the sum stays in ascending c; unrolling keeps 16 independent loads in flight per work-item
+#pragma unroll 16
             for (std::uint32_t c = 0; c < strips; ++c) acc += in.P[cs::id<2>{c, j}];
             if constexpr (Z)
                 out.x[cs::id<1>{j}] = fmaf(scale, acc, in.z[cs::id<1>{j}]);
@@ -124,9 +170,25 @@ struct ColReduce {
                 out.x[cs::id<1>{j}] = scale * acc;
         }
     }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override {
+        this->declare_input(inputs.P, "P", cs::unspecified_shape<2>());
+        if constexpr (Z) this->declare_input(inputs.z, "z", cs::unspecified_shape<1>());
+        this->declare_input(inputs.scale, "scale", cs::unspecified_shape<1>());
+        this->declare_output(outputs.x, "x", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { this->set_shape(outputs.x, this->shape_of(inputs.P)[1]); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t range = inputs.P.shape()[1];
+        return this->create_launch_config(cs::range_desc(cs::range<1>{round_up(range, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
+    }
 };
 
-struct Gemv {
+class Gemv : public cs::Component<Gemv> {
+public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
         cs::Input<DType::f32, 2> B;
@@ -147,6 +209,21 @@ struct Gemv {
             const float sum = cs::reduce_over_group(it.get_group(), acc, cs::plus<float>());
             if (l == 0) out.y[cs::id<1>{i}] = scale * sum;
         }
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override {
+        declare_input(inputs.B, "B", cs::unspecified_shape<2>());
+        declare_input(inputs.x, "x", cs::unspecified_shape<1>());
+        declare_input(inputs.scale, "scale", cs::unspecified_shape<1>());
+        declare_output(outputs.y, "y", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { set_shape(outputs.y, shape_of(inputs.B)[0]); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t range = inputs.B.shape()[0] * kWg;  // a group per row
+        return create_launch_config(cs::range_desc(cs::range<1>{range}), cs::wg_desc(cs::range<1>{kWg}));
     }
 };
 
@@ -242,7 +319,8 @@ __device__ inline void load_f32(const float* p, float (&v)[N]) {
 // =============================================================================
 
 template <DType S>
-struct AhatTPartial {
+class AhatTPartial : public cs::Component<AhatTPartial<S>> {
+public:
     static constexpr int max_work_group_size = kWg;
     using St                                 = Storage<S>;
     static constexpr int kVec                = St::kVec;
@@ -311,6 +389,36 @@ struct AhatTPartial {
             for (std::uint32_t v = 0; v < m; ++v) out.P[cs::id<2>{c, j0 + v}] = acc[v];
         }
     }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    explicit AhatTPartial(int R) : R_{R} {}
+    void declare() override {
+        this->declare_input(inputs.A, "A", cs::unspecified_shape<2>());
+        this->declare_input(inputs.row_scale, "row_scale", cs::unspecified_shape<1>());
+        this->declare_input(inputs.u1, "u1", cs::unspecified_shape<1>());
+        this->declare_input(inputs.v1, "v1", cs::unspecified_shape<1>());
+        this->declare_input(inputs.u2, "u2", cs::unspecified_shape<1>());
+        this->declare_input(inputs.v2, "v2", cs::unspecified_shape<1>());
+        this->declare_input(inputs.y, "y", cs::unspecified_shape<1>());
+        this->declare_input(inputs.R, "R", 1);
+        this->init_input("R", cs::init::fill(u32(static_cast<std::size_t>(R_))));
+        this->declare_output(outputs.P, "P", cs::unspecified_shape<2>());
+    }
+    void prepare_forward() override {
+        const cs::range<2> a = this->shape_of(inputs.A);
+        this->set_shape(outputs.P, (a[0] + R_ - 1) / R_, a[1]);
+    }
+    /// a work-group per (strip of R rows, block of kCols columns)
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const cs::range<2> a = inputs.A.shape();
+        const std::size_t  strips = (a[0] + R_ - 1) / R_, blocks = (a[1] + kCols - 1) / kCols;
+        return this->create_launch_config(cs::range_desc(cs::range<2>{strips, blocks * kWg}), cs::wg_desc(cs::range<2>{1, kWg}));
+    }
+
+private:
+    int R_;
 };
 
 // =============================================================================
@@ -318,7 +426,8 @@ struct AhatTPartial {
 // =============================================================================
 
 template <DType S, int RT>
-struct AhatGemv {
+class AhatGemv : public cs::Component<AhatGemv<S, RT>> {
+public:
     static constexpr int max_work_group_size = kWg;
     using St                                 = Storage<S>;
     static constexpr int kVec                = St::kVec;
@@ -423,6 +532,27 @@ struct AhatGemv {
             const float sum   = (wave_sums[0][tid] + wave_sums[1][tid]) + (wave_sums[2][tid] + wave_sums[3][tid]);
             out.w[cs::id<1>{r0 + tid}] = alpha * sum;
         }
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override {
+        this->declare_input(inputs.A, "A", cs::unspecified_shape<2>());
+        this->declare_input(inputs.row_scale, "row_scale", cs::unspecified_shape<1>());
+        this->declare_input(inputs.u1, "u1", cs::unspecified_shape<1>());
+        this->declare_input(inputs.v1, "v1", cs::unspecified_shape<1>());
+        this->declare_input(inputs.u2, "u2", cs::unspecified_shape<1>());
+        this->declare_input(inputs.v2, "v2", cs::unspecified_shape<1>());
+        this->declare_input(inputs.x, "x", cs::unspecified_shape<1>());
+        this->declare_input(inputs.scale, "scale", cs::unspecified_shape<1>());
+        this->declare_output(outputs.w, "w", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { this->set_shape(outputs.w, this->shape_of(inputs.A)[0]); }
+    /// a work-group of kWg work-items per RT rows (the body needs exactly kWg: four waves)
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t groups = (inputs.A.shape()[0] + RT - 1) / RT;
+        return this->create_launch_config(cs::range_desc(cs::range<1>{groups * kWg}), cs::wg_desc(cs::range<1>{kWg}));
     }
 };
 

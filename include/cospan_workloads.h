@@ -50,6 +50,8 @@ int cs_gemver_time(cs_gemver g, int iters, float* ms_per_run);
 /* `iters` forwards with a device event around every launch: the average milliseconds of each launch of one forward, in
  * launch order (up to `cap` entries); *count = the launches of one forward */
 int cs_gemver_time_launches(cs_gemver g, int iters, float* ms, int cap, int* count);
+/* the model's executor counters: runs, kernel launches, graph captures, graph replays */
+int cs_gemver_stats(cs_gemver g, uint64_t* stats4);
 int cs_gemver_destroy(cs_gemver g);
 
 #ifdef __cplusplus
