@@ -1,13 +1,18 @@
 """Benchmark of the fused-run path on one MI355X: prints ONE JSON line.
 
-Workload (the configuration the metric is quoted on): GEMVER N = 16384, f32, A uniform[-1, 1] (1 GiB), alpha = 1.5,
-beta = 1.2, seed 0, run as ONE fused two-pass model (Â never stored; A read twice). A step is one forward of the model
-(x = beta Â^T y + z, then w = alpha Â x) with every input resident in HBM. `value` is algorithmic HBM bytes per step
-(2 N^2 * 4 + 13 N * 4 = 2 148 335 616 B) over the measured time per step, in GB/s.
+Headline workload (the configuration the metric is quoted on): GEMVER N = 16384, f32, A uniform[-1, 1) (1 GiB),
+alpha = 1.5, beta = 1.2, seed 0, run as the fused two-pass model (Â never stored; A read twice). A step is one forward of
+the model (x = beta Â^T y + z, then w = alpha Â x) with every input resident in HBM; after the second step every step
+replays one captured graph. `value` = algorithmic HBM bytes per step (2 N^2 * 4 + 13 N * 4 = 2 148 335 616 B) / measured
+time per step, in GB/s.
 
-Beside it: the same GEMVER as the reference's four unfused components (B materialised), the per-launch times and the
-roofline of the dominant kernel, and the CPU baseline (the oracle's f32 two-pass port with OpenMP on the host cores, on
-a bounded sample). More than one GPU is not part of this build: --gpus N > 1 exits with a message.
+Beside it (`extra`): the same GEMVER as the reference's four unfused components (B materialised); the NS residual at
+4096^2 fused (one launch) and unfused; the batched ragged GEMVER (512 instances, N log-uniform in [64, 4096]: instances/s,
+p50/p99 per replay, per-byte rate against the fused N=16384 rate); GEMVER N = 32768 with A stored f32, bf16 and e4m3 +
+row scale (speedups) and the NS residual at 8192^2 with bf16 fields. Errors against the FP64 oracle are measured by the
+GPU tests (tests/test_*_gpu.py), not here. The roofline object is for the dominant kernel of the headline workload; the
+CPU baseline is the oracle's f32 two-pass port with OpenMP on the host cores, on a bounded sample. More than one GPU is not
+part of this build: --gpus N > 1 exits with a message.
 """
 import argparse
 import ctypes as C
@@ -28,8 +33,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gemver_bytes_fused(n):
-    return 2 * n * n * 4 + 13 * n * 4
+def gemver_bytes_fused(n, elem=4):
+    return 2 * n * n * elem + 13 * n * 4
 
 
 def gemver_bytes_plain(n, R):
@@ -62,7 +67,32 @@ class Gemver:
         self.check(self.w.cs_gemver_destroy(self.h))
 
 
+class Ns:
+    def __init__(self, style, n, store=0, ty=16):
+        from cospanmdao_amd import check, workloads
+
+        self.w, self.check = workloads, check
+        self.h = C.c_void_p()
+        check(workloads.cs_ns_create(C.byref(self.h), style, store, n, n, 1.0 / n, 1e-3, 0, ty))
+
+    def run(self):
+        self.check(self.w.cs_ns_run(self.h))
+
+    def wait(self):
+        self.check(self.w.cs_ns_wait(self.h))
+
+    def launches(self, iters):
+        ms = (C.c_float * 16)()
+        cnt = C.c_int()
+        self.check(self.w.cs_ns_time_launches(self.h, iters, ms, 16, C.byref(cnt)))
+        return [ms[k] for k in range(cnt.value)]
+
+    def close(self):
+        self.check(self.w.cs_ns_destroy(self.h))
+
+
 def timed_steps(model, steps, warmup):
+    """Wall-clock ms per step: `warmup` untimed steps, then `steps` steps bracketed by device synchronisation."""
     from cospanmdao_amd import check, hip
 
     for _ in range(warmup):
@@ -75,6 +105,83 @@ def timed_steps(model, steps, warmup):
     model.wait()
     check(hip.cs_hip_device_synchronize())
     return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9
+
+
+def bench_ns(extra, steps):
+    for n, store, tag in ((4096, 0, "ns4096_f32"), (8192, 0, "ns8192_f32"), (8192, 1, "ns8192_bf16")):
+        cells = n * n
+        fbytes = cells * (3 * (2 if store else 4) + 3 * 4)
+        rec = {}
+        for style, name in ((0, "fused"), (1, "unfused")):
+            if store == 1 and style == 1:
+                continue
+            m = Ns(style, n, store)
+            ms = timed_steps(m, steps, 3)
+            ln = m.launches(3)
+            m.close()
+            rec[name] = {"ms": round(ms, 4), "GBps": round(gbs(fbytes, ms), 1), "pct_of_peak": round(100 * gbs(fbytes, ms) / PEAK_GBS, 2),
+                         "launch_ms": [round(t, 4) for t in ln]}
+        if "unfused" in rec:
+            rec["fused_speedup_vs_unfused"] = round(rec["unfused"]["ms"] / rec["fused"]["ms"], 3)
+        rec["algorithmic_bytes"] = fbytes
+        extra[tag] = rec
+        log(f"{tag}: {rec}")
+    extra["ns8192_bf16"]["speedup_vs_f32"] = round(extra["ns8192_f32"]["fused"]["ms"] / extra["ns8192_bf16"]["fused"]["ms"], 3)
+
+
+def bench_precision(extra, steps):
+    n = 32768
+    rec = {}
+    for store, name, elem in ((0, "f32", 4), (1, "bf16", 2), (2, "e4m3", 1)):
+        g = Gemver(TWOPASS, n, 128, store=store)
+        ms = timed_steps(g, steps, 3)
+        ln = g.launches(3)
+        g.close()
+        rec[name] = {"ms": round(ms, 4), "GBps": round(gbs(gemver_bytes_fused(n, elem) + (n * 8 if store == 2 else 0), ms), 1),
+                     "launch_ms": [round(t, 4) for t in ln]}
+    rec["bf16_speedup_vs_f32"] = round(rec["f32"]["ms"] / rec["bf16"]["ms"], 3)
+    rec["e4m3_speedup_vs_f32"] = round(rec["f32"]["ms"] / rec["e4m3"]["ms"], 3)
+    extra["gemver32768_storage"] = rec
+    log(f"gemver N=32768 storage: {rec}")
+
+
+def bench_batch(extra, fused_gbps, reps):
+    from cospanmdao_amd import check, workloads as W
+    import numpy as np
+
+    h = C.c_void_p()
+    check(W.cs_batch_create(C.byref(h), 512, 64, 4096, 1.5, 1.2, 0))
+    sizes = np.empty(512, np.uint32)
+    check(W.cs_batch_sizes(h, sizes.ctypes.data_as(C.POINTER(C.c_uint32)), 512))
+    best = (C.c_uint32 * 3)()
+    cache = os.path.join(os.environ.get("XDG_CACHE_HOME", os.path.join(os.path.expanduser("~"), ".cache")), "cospanmdao_amd")
+    os.makedirs(cache, exist_ok=True)
+    t0 = time.perf_counter()
+    check(W.cs_batch_autotune(h, 3, os.path.join(cache, "autotune.txt").encode(), best))
+    tune_s = time.perf_counter() - t0
+    for _ in range(3):
+        check(W.cs_batch_run(h))
+    check(W.cs_batch_wait(h))
+    each = (C.c_float * reps)()
+    check(W.cs_batch_time_each(h, reps, each))
+    ms = np.array(list(each), np.float64)
+    st = (C.c_uint64 * 4)()
+    check(W.cs_batch_stats(h, st))
+    check(W.cs_batch_destroy(h))
+    n = sizes.astype(np.float64)
+    nbytes = float(np.sum(2 * n * n * 4 + 13 * n * 4))
+    mean = float(ms.mean())
+    rate = gbs(nbytes, mean)
+    rec = {"instances": 512, "instances_per_s": round(512 / (mean * 1e-3), 1), "ms_mean": round(mean, 4),
+           "ms_p50": round(float(np.percentile(ms, 50)), 4), "ms_p99": round(float(np.percentile(ms, 99)), 4),
+           "algorithmic_bytes": int(nbytes), "GBps": round(rate, 1), "per_byte_rate_vs_fused_16384": round(rate / fused_gbps, 3),
+           "params_R_RT_G": list(best), "autotune_s": round(tune_s, 2), "replays": reps, "executor_stats": list(st)}
+    extra["batched_ragged"] = rec
+    log(f"batched: {rec}")
 
 
 def cpu_baseline(n, R, budget_s=12.0):
@@ -114,7 +221,7 @@ def main():
     ap.add_argument("--R", type=int, default=128)
     ap.add_argument("--rt2", type=int, default=8)
     ap.add_argument("--style", default="twopass", choices=["twopass", "twopass_1"])
-    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -131,9 +238,8 @@ def main():
     launch_ms = g.launches(max(5, a.steps // 2))
     g.close()
     fused_bytes = gemver_bytes_fused(n)
-    value = fused_bytes / (ms * 1e-3) / 1e9
-    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step, {value:.1f} GB/s; launches (us): "
-        f"{[round(t * 1e3, 1) for t in launch_ms]}")
+    value = gbs(fused_bytes, ms)
+    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step, {value:.1f} GB/s; launches (us): {[round(t * 1e3, 1) for t in launch_ms]}")
 
     # the dominant kernel: pass 1 (A + u1 v1 u2 v2 y) or pass 2 (A + u1 v1 u2 v2 x + w)
     if style == TWOPASS and len(launch_ms) == 3:
@@ -143,24 +249,24 @@ def main():
         kms = launch_ms[k]
     else:
         kbytes, kname, kms = fused_bytes, "gemver fused two-pass (one launch)", launch_ms[0]
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_GBS, 4), "traffic": None, "kernel": kname,
-                "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}
+    achieved = gbs(kbytes, kms)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
+                "traffic": None, "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}
 
-    extra = {"fused_ms": round(ms, 4), "fused_pct_of_peak": round(100 * value / PEAK_GBS, 2),
-             "launch_ms": [round(t, 4) for t in launch_ms]}
-    if not a.no_plain:
+    extra = {"fused_ms": round(ms, 4), "fused_pct_of_peak": round(100 * value / PEAK_GBS, 2), "launch_ms": [round(t, 4) for t in launch_ms]}
+    if not a.quick:
         p = Gemver(UNFUSED, n, 64)
         pms = timed_steps(p, max(3, a.steps // 2), 2)
         p.close()
-        extra.update({"plain_ms": round(pms, 4), "plain_GBps_own_bytes": round(gemver_bytes_plain(n, 64) / (pms * 1e-3) / 1e9, 1),
-                      "plain_GBps_fused_bytes": round(fused_bytes / (pms * 1e-3) / 1e9, 1),
-                      "fused_speedup_vs_plain": round(pms / ms, 3)})
+        extra["plain"] = {"ms": round(pms, 4), "GBps_own_bytes": round(gbs(gemver_bytes_plain(n, 64), pms), 1),
+                          "GBps_fused_bytes": round(gbs(fused_bytes, pms), 1), "fused_speedup_vs_plain": round(pms / ms, 3)}
         log(f"plain (4 components, B materialised): {pms * 1e3:.1f} us/step; fused speedup {pms / ms:.2f}x")
+        bench_ns(extra, a.steps)
+        bench_batch(extra, value, max(30, a.steps * 3))
+        bench_precision(extra, a.steps)
 
     cpu = None
-    if not a.no_cpu:
+    if not (a.no_cpu or a.quick):
         cpu = cpu_baseline(n, R)
         log(f"cpu baseline: {cpu}")
 

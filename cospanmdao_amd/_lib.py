@@ -77,6 +77,26 @@ WORKLOADS_API = [
     ("cs_gemver_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
     ("cs_gemver_stats", _I, [_P, C.POINTER(C.c_uint64)]),
     ("cs_gemver_destroy", _I, [_P]),
+    ("cs_ns_create", _I, [C.POINTER(_P), _I, _I, _S, _S, C.c_double, C.c_double, _U32, _I]),
+    ("cs_ns_run", _I, [_P]),
+    ("cs_ns_wait", _I, [_P]),
+    ("cs_ns_read", _I, [_P, C.c_char_p, _P, _S]),
+    ("cs_ns_write", _I, [_P, C.c_char_p, _P, _S]),
+    ("cs_ns_time", _I, [_P, _I, C.POINTER(_F)]),
+    ("cs_ns_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
+    ("cs_ns_destroy", _I, [_P]),
+    ("cs_batch_create", _I, [C.POINTER(_P), _U32, _U32, _U32, _F, _F, _U32]),
+    ("cs_batch_sizes", _I, [_P, C.POINTER(_U32), _S]),
+    ("cs_batch_set_params", _I, [_P, _U32, _U32, _U32]),
+    ("cs_batch_get_params", _I, [_P, C.POINTER(_U32)]),
+    ("cs_batch_autotune", _I, [_P, _I, C.c_char_p, C.POINTER(_U32)]),
+    ("cs_batch_run", _I, [_P]),
+    ("cs_batch_wait", _I, [_P]),
+    ("cs_batch_read", _I, [_P, C.c_char_p, _P, _S]),
+    ("cs_batch_time", _I, [_P, _I, C.POINTER(_F)]),
+    ("cs_batch_time_each", _I, [_P, _I, C.POINTER(_F)]),
+    ("cs_batch_stats", _I, [_P, C.POINTER(C.c_uint64)]),
+    ("cs_batch_destroy", _I, [_P]),
 ]
 
 

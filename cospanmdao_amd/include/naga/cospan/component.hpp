@@ -17,6 +17,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -110,6 +111,10 @@ struct ComponentAccess;
 // -----------------------------------------------------------------------------
 // ComponentBase
 // -----------------------------------------------------------------------------
+// The component classes are host-and-device (a CS_COMPONENTS region): the user's classes, defined in such a region so
+// that their static compute_forward is callable from the fused kernel, override these hooks with the same attributes.
+// Nothing here is ever compiled for the device except what a kernel calls.
+CS_COMPONENTS_BEGIN
 class ComponentBase {
 public:
     virtual ~ComponentBase() = default;
@@ -133,6 +138,15 @@ protected:
         add_port(&w, std::move(name), detail::IOKind::output, DT, R, shape,
                  [&w](void* p, const std::size_t* s, const std::size_t* st) { detail::WrapperAccess::point(w, p, s, st); });
     }
+    /// An output the component keeps as device state between forwards (work counters, look-back status words): zeroed
+    /// whenever it is (re)allocated, and left zeroed by the kernels that use it, so every forward (and every replay of a
+    /// captured graph) starts from zero without a memset.
+    template <DType DT, int R>
+    void declare_scratch(Output<DT, R>& w, std::string name, ShapeSpec shape = {}) {
+        declare_output(w, name, shape);
+        ports_.back().inst->var->scratch = true;
+    }
+
     /// The initial value of one of this component's inputs (the last declaration wins; ignored when the input is fed).
     void init_input(std::string_view name, InitSpec init) {
         Port&                     p = port_named(name, "init_input");
@@ -248,7 +262,13 @@ public:
             if (!v.shape_known())
                 throw std::invalid_argument("naga::cospan: " + describe() + ": '" + p.name + "' has an unknown shape " + v.shape_text() +
                                             " after prepare_forward");
-            if (p.kind == detail::IOKind::output) v.ensure_allocation();
+            if (p.kind == detail::IOKind::output && v.ensure_allocation() && v.scratch) {
+                // scratch state starts at zero; the kernels that use it leave it zeroed (graph-safe, no memset node)
+                m.exec.init();
+                m.exec.before_init(v);
+                detail::hip_check(cs_hip_fill_u8_async(v.data, 0, v.capacity, m.exec.stream(0)), "zeroing of '" + v.name + "'");
+                m.exec.after_init(v);
+            }
             std::size_t stride[3];
             detail::row_major_strides(v, stride);
             p.point(v.data, v.dims, stride);
@@ -337,6 +357,40 @@ protected:
     }
 };
 
+/// What a custom-op body receives (reference kernel.hpp:56-68, "native backend work"): the stream its work goes on, and
+/// the device pointers of its ports.
+struct CustomOp {
+    cs_hip_stream stream;
+    template <class W>
+    auto device_ptr(const W& w) const {
+        return raw::data(w);
+    }
+};
+
+namespace detail {
+template <class X>
+class CustomLaunch : public LaunchDescription {
+public:
+    typename X::Inputs  in;
+    typename X::Outputs out;
+};
+}  // namespace detail
+
+/// A CUSTOM-OP component (reference component.hpp:1490-1503): `static void compute_forward(cs::CustomOp&, const Inputs&,
+/// const Outputs&)` enqueues native work (a C-boundary call, a library kernel) on op.stream. No define_forward; not fusible.
+template <class X>
+class CustomComponent : public ComponentBase {
+public:
+    std::unique_ptr<LaunchDescription> describe_launch() override {
+        X&   self = static_cast<X&>(*this);
+        auto d    = std::make_unique<detail::CustomLaunch<X>>();
+        d->in     = self.inputs;
+        d->out    = self.outputs;
+        return d;
+    }
+};
+CS_COMPONENTS_END
+
 namespace detail {
 
 /// The segment layout of a bare component: one segment of one stage (the native launch).
@@ -364,6 +418,34 @@ public:
 private:
     std::unique_ptr<X> comp_;
     GroupLauncher<X>   launcher_;
+};
+
+/// The leaf of a custom-op component: its launch is the body itself, run on the executor's chosen stream.
+template <class X>
+class SoloCustomDriver : public Leaf {
+public:
+    explicit SoloCustomDriver(std::unique_ptr<X> c) : comp_{std::move(c)} {}
+    LaunchRecord prepare(ModelMeta& m) override {
+        reads.clear();
+        writes.clear();
+        comp_->precompute(m, reads, writes);
+        std::unique_ptr<LaunchDescription> d  = comp_->describe_launch();
+        auto*                              cl = dynamic_cast<CustomLaunch<X>*>(d.get());
+        LaunchRecord                       rec;
+        rec.name = path;
+        static_assert(std::is_trivially_copyable_v<typename X::Inputs> && std::is_trivially_copyable_v<typename X::Outputs>);
+        rec.args.resize(sizeof(typename X::Inputs) + sizeof(typename X::Outputs));
+        std::memcpy(rec.args.data(), &cl->in, sizeof cl->in);
+        std::memcpy(rec.args.data() + sizeof cl->in, &cl->out, sizeof cl->out);
+        rec.custom = [in = cl->in, out = cl->out](cs_hip_stream s) {
+            CustomOp op{s};
+            X::compute_forward(op, in, out);
+        };
+        return rec;
+    }
+
+private:
+    std::unique_ptr<X> comp_;
 };
 
 }  // namespace detail

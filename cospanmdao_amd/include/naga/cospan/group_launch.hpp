@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 #include <cstdint>
 #include <stdexcept>
@@ -94,11 +95,17 @@ struct LaunchRecord {
     std::uint32_t          grid[3]  = {0, 0, 0};
     std::uint32_t          block[3] = {1, 1, 1};
     std::size_t            lds      = 0;
-    std::vector<std::byte> args;  ///< the one by-value kernel argument (GroupArgs<X...>)
+    std::vector<std::byte> args;  ///< the one by-value kernel argument (GroupArgs<X...>), or a custom op's ports
     std::string            name;
+    /// A custom-op launch (CustomComponent): native work enqueued on the stream instead of a group kernel.
+    std::function<void(cs_hip_stream)> custom;
 
-    bool empty() const { return kernel == nullptr; }
+    bool empty() const { return kernel == nullptr && !custom; }
     void submit(cs_hip_stream s) const {
+        if (custom) {
+            custom(s);
+            return;
+        }
         if (empty()) return;
         void* a[] = {const_cast<std::byte*>(args.data())};
         hip_check(cs_hip_launch(kernel, grid, block, lds, a, s), "launch of '" + name + "'");

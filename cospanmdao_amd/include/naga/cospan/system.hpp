@@ -96,7 +96,15 @@ public:
     void add_subsystem(SubsystemDesc<T, A...> desc) {
         detail::ModelMeta& m    = meta("add_subsystem");
         const std::string  full = detail::ModelMeta::join(path_, desc.name);
-        if constexpr (std::is_base_of_v<ComponentBase, T>) {
+        if constexpr (std::is_base_of_v<CustomComponent<T>, T>) {
+            std::unique_ptr<T> c = detail::make_subsystem<T>(desc.args);
+            detail::ComponentAccess::bind(*c, full, &m);
+            c->declare();
+            auto drv  = std::make_unique<detail::SoloCustomDriver<T>>(std::move(c));
+            drv->path = full;
+            m.leaves.push_back(drv.get());
+            leaves_.push_back(std::move(drv));
+        } else if constexpr (std::is_base_of_v<ComponentBase, T>) {
             std::unique_ptr<T> c = detail::make_subsystem<T>(desc.args);
             detail::ComponentAccess::bind(*c, full, &m);
             c->declare();

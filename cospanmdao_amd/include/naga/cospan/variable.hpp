@@ -180,6 +180,7 @@ struct VarMeta {
     bool                    pending      = false;  ///< the initializer has to (re)run before the next reader
     bool                    has_producer = false;  ///< an output writes it
     bool                    has_value    = false;  ///< written at least once (by an initializer or a producer)
+    bool                    scratch      = false;  ///< a declare_scratch output: zeroed at every (re)allocation
 
     void*       data     = nullptr;
     std::size_t capacity = 0;  ///< bytes allocated
@@ -226,15 +227,17 @@ struct VarMeta {
         data     = nullptr;
         capacity = 0;
     }
-    /// Allocation of at least bytes() (the old contents are not kept: sizes change only at materialization).
-    void ensure_allocation() {
+    /// Allocation of at least bytes() (the old contents are not kept: sizes change only at materialization). True when
+    /// a new allocation was made.
+    bool ensure_allocation() {
         const std::size_t need = bytes();
-        if (need <= capacity && data) return;
+        if (need <= capacity && data) return false;
         release();
         void* p = nullptr;
         hip_check(cs_hip_malloc(&p, need < 256 ? 256 : need), "allocation of '" + name + "' (" + std::to_string(need) + " bytes)");
         data     = p;
         capacity = need < 256 ? 256 : need;
+        return true;
     }
 };
 

@@ -1,0 +1,364 @@
+#pragma once
+
+// =============================================================================
+// cospanmdao_amd/workloads/batch_components.hpp — batched ragged GEMVER: K independent instances in one persistent launch
+// =============================================================================
+//
+// Instance k has size N_k (any N >= 1): A_k (N_k x N_k f32, leading dimension ld_k = N_k rounded up to 4, so rows are
+// 16-byte aligned), vectors u1 v1 u2 v2 y z of N_k, outputs x_k = beta Â_k^T y_k + z_k and w_k = alpha Â_k x_k. Every
+// instance's arrays sit back to back in one allocation per array; where each starts comes from a DEVICE exclusive scan:
+//
+//   BatchPlan      (index range over K) per instance six counts, stored column-major [6][K]:
+//                    0 V  = N_k                                   vector elements          (x, w, u1 .. z offsets)
+//                    1 A4 = N_k * ld_k / 4                        16-byte units of A_k     (A offsets)
+//                    2 P  = strips_k * N_k                        strip partials           (P offsets)
+//                    3 U1 = strips_k * blocks_k                   pass-1 work units
+//                    4 UR = blocks_k                              column-reduction units
+//                    5 U2 = ceil(N_k / RT)                        pass-2 work units
+//                  (strips_k = ceil(N_k / R), blocks_k = ceil(N_k / 1024))
+//   ColumnScan     (custom op) ONE u32 exclusive scan of the 6K counts (cs_hip_build_column: decoupled look-back, wrap
+//                  flag): column c's offsets are scan[cK + k] - scan[cK]; column 0 is the int32 scan of the sizes.
+//   RaggedGemver   (persistent) a grid of G work-groups fetches work units by ticket (start order) until none is left:
+//                  first every pass-1 tile of every instance, then every column reduction, then every pass-2 row group. A
+//                  reduction of instance k waits until all of k's pass-1 tiles are done, a pass-2 unit until all of k's
+//                  reductions are: every wait is on units with earlier tickets, held by work-groups that are running them
+//                  and never wait on later ones, so the queue drains without co-residency. The last work-group to leave
+//                  zeroes the counters (graph replays need no memset).
+// The arithmetic of a tile is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
+// the reference components' f32 x for the same R bit for bit. R, RT (4 or 8) and G are run-time parameters (the
+// autotuner's knobs).
+
+#include <cstdint>
+#include <memory>
+
+#include "naga/cospan/cospan.hpp"
+
+namespace batch {
+
+namespace cs = naga::cospan;
+using cs::DType;
+
+inline constexpr int kWg    = 256;
+inline constexpr int kCols  = kWg * 4;  // columns of one pass-1 tile / one reduction unit / one pass-2 chunk
+inline constexpr int kMaxRT = 8;
+
+__host__ __device__ inline std::uint32_t ceil_div(std::uint32_t a, std::uint32_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline std::uint32_t ld_of(std::uint32_t n) { return (n + 3u) & ~3u; }
+
+/// Knobs shared between the model's owner (the autotuner) and the persistent component: read at every describe.
+struct Knobs {
+    int groups = 1024;
+};
+
+CS_COMPONENTS_BEGIN
+
+/// Per-instance counts, [6][K] column-major. params = {R, RT, G}.
+class BatchPlan : public cs::Component<BatchPlan> {
+public:
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes;
+        cs::Input<DType::u32, 1> params;
+    };
+    struct Outputs {
+        cs::Output<DType::u32, 1> counts;
+    };
+    __device__ static void compute_forward(cs::id<1> k, const Inputs& in, const Outputs& out) {
+        const std::size_t   K  = in.sizes.size();
+        const std::uint32_t n  = in.sizes[k];
+        const std::uint32_t R  = in.params[std::size_t{0}], RT = in.params[std::size_t{1}];
+        const std::uint32_t st = ceil_div(n, R), bl = ceil_div(n, kCols);
+        out.counts[0 * K + k] = n;
+        out.counts[1 * K + k] = n * ld_of(n) / 4u;
+        out.counts[2 * K + k] = st * n;
+        out.counts[3 * K + k] = st * bl;
+        out.counts[4 * K + k] = bl;
+        out.counts[5 * K + k] = ceil_div(n, RT);
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.params, "params", 3);
+        declare_output(outputs.counts, "counts", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { set_shape(outputs.counts, 6 * shape_of(inputs.sizes)[0]); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override { return create_launch_config(cs::range_desc(inputs.sizes.shape())); }
+};
+
+/// offs[0..M] = exclusive scan of counts (u32, wraps flagged), one launch of the library's decoupled look-back scan.
+class ColumnScan : public cs::CustomComponent<ColumnScan> {
+public:
+    struct Inputs {
+        cs::Input<DType::u32, 1> counts;
+    };
+    struct Outputs {
+        cs::Output<DType::u32, 1> offs;
+        cs::Output<DType::u32, 1> extents;
+        cs::Output<DType::u32, 1> result;     ///< [total, flag] of the column build
+        cs::Output<DType::u8, 1>  workspace;  ///< the scan's look-back status words (scratch: left zeroed)
+    };
+    static void compute_forward(cs::CustomOp& op, const Inputs& in, const Outputs& out) {
+        cs::detail::hip_check(cs_hip_build_column(1, op.device_ptr(in.counts), in.counts.size(), 1, 0, op.device_ptr(out.extents),
+                                                  op.device_ptr(out.offs), op.device_ptr(out.result), op.device_ptr(out.workspace), op.stream),
+                              "batch offsets scan");
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.counts, "counts", cs::unspecified_shape<1>());
+        declare_output(outputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_output(outputs.extents, "extents", cs::unspecified_shape<1>());
+        declare_output(outputs.result, "result", 2);
+        declare_scratch(outputs.workspace, "workspace", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override {
+        const std::size_t m = shape_of(inputs.counts)[0];
+        set_shape(outputs.offs, m + 1);
+        set_shape(outputs.extents, m);
+        set_shape(outputs.workspace, cs_hip_scan_workspace_bytes(m));
+    }
+};
+
+/// The persistent work-queue kernel.
+class RaggedGemver : public cs::Component<RaggedGemver> {
+public:
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::f32, 1> A, u1, v1, u2, v2, y, z;
+        cs::Input<DType::f32, 1> alpha, beta;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> P, x, w;
+        cs::Output<DType::u32, 1> state;  ///< [ticket, finished, pass-1 done per instance, reductions done per instance]
+    };
+
+    struct Cols {
+        const std::uint32_t* s;  // the scan
+        std::uint32_t        K;
+        __device__ std::uint32_t off(int c, std::uint32_t k) const { return s[c * K + k] - s[c * K]; }
+        __device__ std::uint32_t total(int c) const { return s[(c + 1) * K] - s[c * K]; }
+        /// the instance whose range of column c holds unit u: the largest k with off(c, k) <= u
+        __device__ std::uint32_t find(int c, std::uint32_t u) const {
+            std::uint32_t lo = 0, hi = K - 1;
+            while (lo < hi) {
+                const std::uint32_t mid = (lo + hi + 1) / 2;
+                if (off(c, mid) <= u)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            return lo;
+        }
+    };
+
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const Cols          cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t K  = cols.K;
+        const std::uint32_t R  = in.params[std::size_t{0}], RT = in.params[std::size_t{1}];
+        const std::uint32_t T1 = cols.total(3), TR = cols.total(4), T2 = cols.total(5), T = T1 + TR + T2;
+        std::uint32_t*      st = cs::raw::data(out.state);
+        std::uint32_t*      p1_done  = st + 2;
+        std::uint32_t*      red_done = st + 2 + K;
+        const std::uint32_t tid      = static_cast<std::uint32_t>(it.get_local_id(0));
+        __shared__ std::uint32_t s_unit;
+
+        for (;;) {
+            if (tid == 0) s_unit = __hip_atomic_fetch_add(&st[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __syncthreads();
+            const std::uint32_t u = s_unit;
+            __syncthreads();
+            if (u >= T) break;
+            if (u < T1) {
+                const std::uint32_t k = cols.find(3, u);
+                pass1(in, out, cols, k, u - cols.off(3, k), R, tid);
+                complete(&p1_done[k], tid);
+            } else if (u < T1 + TR) {
+                const std::uint32_t k = cols.find(4, u - T1);
+                wait_for(&p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
+                reduce(in, out, cols, k, u - T1 - cols.off(4, k), tid);
+                complete(&red_done[k], tid);
+            } else {
+                const std::uint32_t k = cols.find(5, u - T1 - TR);
+                wait_for(&red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
+                const std::uint32_t g = u - T1 - TR - cols.off(5, k);
+                if (RT == 4)
+                    pass2<4>(in, out, cols, k, g, tid);
+                else
+                    pass2<8>(in, out, cols, k, g, tid);
+            }
+        }
+        // leaving: the last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
+        __shared__ std::uint32_t s_last;
+        if (tid == 0) {
+            const std::uint32_t f = __hip_atomic_fetch_add(&st[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            s_last                = f + 1 == static_cast<std::uint32_t>(it.get_group_range(0)) ? 1u : 0u;
+        }
+        __syncthreads();
+        if (s_last)
+            for (std::uint32_t w = tid; w < 2 + 2 * K; w += kWg) __hip_atomic_store(&st[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+
+    // ---- the units ------------------------------------------------------------------------------------------------
+    /// After a unit: every work-item's writes, then the leader's release increment.
+    __device__ static void complete(std::uint32_t* counter, std::uint32_t tid) {
+        __syncthreads();
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    /// Before a unit: the leader waits (acquire) for `target` completions of earlier units (bounded: a corrupted state
+    /// cannot hang the device), then the group.
+    __device__ static void wait_for(std::uint32_t* counter, std::uint32_t target, std::uint32_t tid) {
+        if (tid == 0)
+            for (std::uint32_t spin = 0; __hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target && spin < (1u << 24); ++spin)
+                __builtin_amdgcn_s_sleep(1);
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+
+    /// pass-1 tile `t` of instance k: strip c = t / blocks, column block b = t % blocks.
+    __device__ static void pass1(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t t, std::uint32_t R,
+                                 std::uint32_t tid) {
+        const std::uint32_t n = in.sizes[std::size_t{k}], ld = ld_of(n), blocks = ceil_div(n, kCols);
+        const std::uint32_t c = t / blocks, b = t % blocks;
+        const std::uint32_t j0 = b * kCols + tid * 4;
+        if (j0 >= n) return;
+        const std::uint32_t i0 = c * R, i1 = i0 + R < n ? i0 + R : n;
+        const std::size_t   vo = cols.off(0, k);
+        const float*        A  = cs::raw::data(in.A) + 4 * static_cast<std::size_t>(cols.off(1, k));
+        const float *       u1 = cs::raw::data(in.u1) + vo, *v1 = cs::raw::data(in.v1) + vo, *u2 = cs::raw::data(in.u2) + vo;
+        const float *       v2 = cs::raw::data(in.v2) + vo, *y = cs::raw::data(in.y) + vo;
+        float*              P  = cs::raw::data(out.P) + cols.off(2, k);
+        float               acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (j0 + 4 <= n) {
+            using F4  = float __attribute__((ext_vector_type(4)));
+            const F4 a1 = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
+            const F4 b1 = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
+            const float* ap = A + static_cast<std::size_t>(i0) * ld + j0;
+#pragma unroll 8
+            for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
+                const F4    r  = __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap));
+                const float yi = y[i], p1 = u1[i], p2 = u2[i];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = fmaf(fmaf(p2, b1[e], fmaf(p1, a1[e], r[e])), yi, acc[e]);
+            }
+            for (int e = 0; e < 4; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
+        } else {
+            const std::uint32_t m = n - j0;
+            for (std::uint32_t i = i0; i < i1; ++i)
+                for (std::uint32_t e = 0; e < m; ++e)
+                    acc[e] = fmaf(fmaf(u2[i], v2[j0 + e], fmaf(u1[i], v1[j0 + e], A[static_cast<std::size_t>(i) * ld + j0 + e])), y[i], acc[e]);
+            for (std::uint32_t e = 0; e < m; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
+        }
+    }
+
+    /// reduction unit b of instance k: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
+    __device__ static void reduce(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t b, std::uint32_t tid) {
+        const std::uint32_t n = in.sizes[std::size_t{k}];
+        const std::uint32_t R = in.params[std::size_t{0}], strips = ceil_div(n, R);
+        const std::size_t   vo = cols.off(0, k);
+        const float*        P  = cs::raw::data(out.P) + cols.off(2, k);
+        const float         beta = in.beta[std::size_t{0}];
+        for (std::uint32_t j = b * kCols + tid; j < n && j < (b + 1) * kCols; j += kWg) {
+            float acc = 0.0f;
+#pragma unroll 8
+            for (std::uint32_t c = 0; c < strips; ++c) acc += P[static_cast<std::size_t>(c) * n + j];
+            out.x[std::size_t{vo + j}] = fmaf(beta, acc, in.z[std::size_t{vo + j}]);
+        }
+    }
+
+    /// pass-2 unit g of instance k: rows [g RT, g RT + RT)
+    template <int RT>
+    __device__ static void pass2(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t g, std::uint32_t tid) {
+        const std::uint32_t n = in.sizes[std::size_t{k}], ld = ld_of(n);
+        const std::uint32_t r0 = g * RT, nr = n - r0 < static_cast<std::uint32_t>(RT) ? n - r0 : static_cast<std::uint32_t>(RT);
+        const std::size_t   vo = cols.off(0, k);
+        const float*        A  = cs::raw::data(in.A) + 4 * static_cast<std::size_t>(cols.off(1, k));
+        const float *       u1 = cs::raw::data(in.u1) + vo, *v1 = cs::raw::data(in.v1) + vo, *u2 = cs::raw::data(in.u2) + vo;
+        const float *       v2 = cs::raw::data(in.v2) + vo, *x = cs::raw::data(out.x) + vo;
+        float               acc[RT], p1[RT], p2[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const std::uint32_t i = r0 + (static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u);
+            p1[r]                 = u1[i];
+            p2[r]                 = u2[i];
+            acc[r]                = 0.0f;
+        }
+        using F4         = float __attribute__((ext_vector_type(4)));
+        std::uint32_t j0 = tid * 4;
+        for (; j0 + 4 <= n; j0 += kCols) {
+            const F4 a = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
+            const F4 b = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
+            const F4 xv = F4{x[j0], x[j0 + 1], x[j0 + 2], x[j0 + 3]};
+            F4       t[RT];
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                t[r]                   = __builtin_nontemporal_load(reinterpret_cast<const F4*>(A + static_cast<std::size_t>(r0 + rr) * ld + j0));
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[r] = fmaf(fmaf(p2[r], b[e], fmaf(p1[r], a[e], t[r][e])), xv[e], acc[r]);
+        }
+        if (j0 < n)
+            for (int r = 0; r < RT; ++r) {
+                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                for (std::uint32_t j = j0; j < n; ++j)
+                    acc[r] = fmaf(fmaf(p2[r], v2[j], fmaf(p1[r], v1[j], A[static_cast<std::size_t>(r0 + rr) * ld + j])), x[j], acc[r]);
+            }
+        __shared__ float ws[kWg / 64][kMaxRT];
+        const int        wv = static_cast<int>(tid) / 64;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const float v = cs::wave::reduce(acc[r], cs::plus<float>{});
+            if (cs::wave::lane_id() == 0) ws[wv][r] = v;
+        }
+        __syncthreads();
+        if (tid < nr) {
+            const float alpha = in.alpha[std::size_t{0}];
+            out.w[std::size_t{vo + r0 + tid}] = alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
+        }
+        __syncthreads();  // ws is reused by the next unit
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.params, "params", 3);
+        declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
+                               {&inputs.y, "y"}, {&inputs.z, "z"}})
+            declare_input(*w, name, cs::unspecified_shape<1>());
+        declare_input(inputs.alpha, "alpha", 1);
+        declare_input(inputs.beta, "beta", 1);
+        declare_output(outputs.P, "P", cs::unspecified_shape<1>());
+        declare_output(outputs.x, "x", cs::unspecified_shape<1>());
+        declare_output(outputs.w, "w", cs::unspecified_shape<1>());
+        declare_scratch(outputs.state, "state", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override {
+        const std::size_t K = shape_of(inputs.sizes)[0];
+        set_shape(outputs.x, shape_of(inputs.u1)[0]);
+        set_shape(outputs.w, shape_of(inputs.u1)[0]);
+        set_shape(outputs.state, 2 + 2 * K);
+    }
+    /// G persistent work-groups (the grid): read from the knobs the owner shares (the autotuner's third knob)
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->groups) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+    explicit RaggedGemver(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
+
+private:
+    std::shared_ptr<Knobs> knobs_;
+};
+
+CS_COMPONENTS_END
+
+}  // namespace batch

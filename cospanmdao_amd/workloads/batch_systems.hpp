@@ -1,0 +1,95 @@
+#pragma once
+
+// =============================================================================
+// cospanmdao_amd/workloads/batch_systems.hpp — the batched ragged GEMVER as a System
+// =============================================================================
+//
+// Leaves: plan (BatchPlan) -> scan (ColumnScan) -> gemver (RaggedGemver, persistent). Ports: inputs sizes, params
+// {R, RT, G}, A (all instances back to back, instance k at 16-byte unit offs[K + k] - offs[K]), u1 v1 u2 v2 y z (back to
+// back, instance k at offs[k]), alpha, beta; outputs offs (the device scan, 6K + 1 entries), result ([total, flag] of the
+// scan), x, w.
+//
+// Inputs: instance k's A(i, j) = hash_unit(seed_A, a_k + i N_k + j) with a_k = sum_{m<k} N_m^2 and its vectors
+// u1_k[i] = hash_unit(seed_u1, v_k + i) with v_k = sum_{m<k} N_m: one flat hash stream per array (seeds as in
+// examples/gemver-chain/common/chain_data.hpp:32), generated on the device.
+
+#include <cstdint>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "naga/cospan/cospan.hpp"
+#include "batch_components.hpp"
+
+namespace batch {
+
+/// Host-side layout of the instances (what the allocations are sized by; the kernels use the device scan's offsets).
+struct Layout {
+    std::vector<std::uint32_t> sizes;
+    std::uint64_t              vec_total = 0, a4_total = 0, p_max = 0;
+    explicit Layout(std::vector<std::uint32_t> s, std::uint32_t min_R) : sizes{std::move(s)} {
+        for (std::uint32_t n : sizes) {
+            vec_total += n;
+            a4_total += static_cast<std::uint64_t>(n) * ld_of(n) / 4;
+            p_max += static_cast<std::uint64_t>(ceil_div(n, min_R)) * n;
+        }
+    }
+};
+
+/// The A initializer is a kernel of the library that builds the model (it needs the per-instance layout).
+using AInit = std::function<cs::InitSpec(const Layout&)>;
+
+inline cs::InitSpec hash_stream(std::uint32_t seed, std::size_t n) {
+    return cs::init::custom<DType::f32, 1>(
+        [seed, n](cs_hip_stream s, cs::Initializer<DType::f32, 1> v) {
+            cs::detail::hip_check(cs_hip_fill_hash_f32(seed, cs::raw::data(v), 1, n, n, s), "hash initializer");
+        },
+        cs::Shape{n});
+}
+
+class BatchedGemver : public cs::System {
+public:
+    static constexpr std::uint32_t kMinR = 32;  // the smallest strip the autotuner tries: P is sized for it
+
+    BatchedGemver(std::shared_ptr<Layout> layout, std::shared_ptr<Knobs> knobs, std::vector<std::uint32_t> params, float alpha, float beta,
+                  std::uint32_t base_seed, AInit a_init)
+        : layout_{std::move(layout)}, knobs_{std::move(knobs)}, params_{std::move(params)}, alpha_{alpha}, beta_{beta}, seed_{base_seed},
+          a_init_{std::move(a_init)} {}
+
+    void setup() override {
+        add_subsystem(cs::subsystem_desc<BatchPlan>("plan"));
+        add_subsystem(cs::subsystem_desc<ColumnScan>("scan"));
+        add_subsystem(cs::subsystem_desc<RaggedGemver>("gemver", knobs_));
+        connect("plan.counts", "scan.counts");
+        connect("scan.offs", "gemver.offs");
+        tie_inputs<DType::u32, 1>({"plan.sizes", "gemver.sizes"}, "sizes");
+        tie_inputs<DType::u32, 1>({"plan.params", "gemver.params"}, "params", cs::shape(3));
+        for (const char* v : {"A", "u1", "v1", "u2", "v2", "y", "z", "alpha", "beta"})
+            promote<DType::f32, 1>(std::string{"gemver."} + v, v);
+        promote<DType::u32, 1>("scan.offs", "offs");
+        promote<DType::u32, 1>("scan.result", "result");
+        promote<DType::u32, 1>("plan.counts", "counts");
+        promote<DType::f32, 1>("gemver.x", "x");
+        promote<DType::f32, 1>("gemver.w", "w");
+        shape_hook("gemver.P", cs::dim::fixed_at_run(layout_->p_max));
+
+        const std::uint32_t sb = seed_ * 16u;
+        init_input("sizes", cs::init::from_host(layout_->sizes));
+        init_input("params", cs::init::from_host(params_));
+        init_input("A", a_init_(*layout_));
+        const char* vecs[6] = {"u1", "v1", "u2", "v2", "y", "z"};
+        for (std::uint32_t i = 0; i < 6; ++i) init_input(vecs[i], hash_stream(sb + 2 + i, layout_->vec_total));
+        init_input("alpha", cs::init::fill(alpha_, {1}));
+        init_input("beta", cs::init::fill(beta_, {1}));
+    }
+
+private:
+    std::shared_ptr<Layout>    layout_;
+    std::shared_ptr<Knobs>     knobs_;
+    std::vector<std::uint32_t> params_;
+    float                      alpha_, beta_;
+    std::uint32_t              seed_;
+    AInit                      a_init_;
+};
+
+}  // namespace batch

@@ -44,6 +44,7 @@ __host__ __device__ inline std::uint32_t u32(std::size_t v) { return static_cast
 
 __device__ inline float rank2(float a, float u1i, float v1j, float u2i, float v2j) { return fmaf(u2i, v2j, fmaf(u1i, v1j, a)); }
 
+CS_COMPONENTS_BEGIN
 // =============================================================================
 // The reference's four components
 // =============================================================================
@@ -555,5 +556,7 @@ public:
         return this->create_launch_config(cs::range_desc(cs::range<1>{groups * kWg}), cs::wg_desc(cs::range<1>{kWg}));
     }
 };
+
+CS_COMPONENTS_END
 
 }  // namespace gemver

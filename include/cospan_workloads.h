@@ -54,6 +54,60 @@ int cs_gemver_time_launches(cs_gemver g, int iters, float* ms, int cap, int* cou
 int cs_gemver_stats(cs_gemver g, uint64_t* stats4);
 int cs_gemver_destroy(cs_gemver g);
 
+/* ---- incompressible Navier-Stokes residual ---------------------------------------------------------------------------
+ * replaces: the fused / unfused variants of examples/ns-residual (systems.hpp:88-140, components.hpp:636-852, driver
+ * main.cpp:159-199) with this build's incompressible physics on a periodic nx x ny grid:
+ *   Ru = u u_x + v u_y + p_x - nu lap u,  Rv = u v_x + v v_y + p_y - nu lap v,  Rc = u_x + v_y  (central differences),
+ * then the FP64 sums of squares ("sq", 3 doubles) and L2 norms ("norm": |Ru|, |Rv|, |Rc|, all three; 4 doubles).
+ *   CS_NS_FUSED    one launch: the tile stencil, a global barrier, the fixed-order reduction of the tile partials
+ *   CS_NS_UNFUSED  four launches: derivatives to global memory, combine, squares, norm
+ * store: 0 f32 fields, 1 bf16 fields (arithmetic f32, norms FP64). tile_rows: rows per work-group of the fused stencil.
+ * Fields: u, v, p from 8 random Fourier modes each (seeded), generated on the device. Names for cs_ns_read: "u" "v" "p"
+ * (as stored), "Ru" "Rv" "Rc" (f32), "sq" "norm" (f64).
+ */
+#define CS_NS_FUSED 0
+#define CS_NS_UNFUSED 1
+
+typedef struct cs_ns_s* cs_ns;
+
+int cs_ns_create(cs_ns* m, int style, int store, size_t nx, size_t ny, double h, double nu, uint32_t seed, int tile_rows);
+int cs_ns_run(cs_ns m);
+int cs_ns_wait(cs_ns m);
+int cs_ns_read(cs_ns m, const char* name, void* host, size_t bytes);
+/* replaces a field ("u", "v", "p") with host values in its storage format before the next run (update_input) */
+int cs_ns_write(cs_ns m, const char* name, const void* host, size_t bytes);
+int cs_ns_time(cs_ns m, int iters, float* ms_per_run);
+int cs_ns_time_launches(cs_ns m, int iters, float* ms, int cap, int* count);
+int cs_ns_destroy(cs_ns m);
+
+/* ---- batched ragged GEMVER -------------------------------------------------------------------------------------------
+ * No reference counterpart as one launch (the reference refuses ragged ports inside fused groups,
+ * cospanmdao/include/naga/cospan/fused_component.hpp:207-213); the offsets come from the column build it uses for
+ * ragged shapes (shape_hook.hpp:215-348). `count` instances with N_k = round(lo (hi/lo)^U_k) (U_k from the hash (seed, k));
+ * each x_k = beta Â_k^T y_k + z_k, w_k = alpha Â_k x_k as GEMVER above. One forward = a plan kernel, ONE device scan of
+ * the per-instance counts, ONE persistent work-queue launch; runs from the second on replay one captured graph.
+ * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (6 x count), "offs" (6 x count + 1: the scan),
+ * "result" ([total, flag] of the scan), "x", "w", "u1" .. "z" (instances back to back).
+ * params: R (strip rows, >= 32), RT (pass-2 rows: 4 or 8), G (persistent work-groups).
+ */
+typedef struct cs_batch_s* cs_batch;
+
+int cs_batch_create(cs_batch* b, uint32_t count, uint32_t lo, uint32_t hi, float alpha, float beta, uint32_t base_seed);
+int cs_batch_sizes(cs_batch b, uint32_t* sizes, size_t count);
+int cs_batch_set_params(cs_batch b, uint32_t R, uint32_t RT, uint32_t groups);
+int cs_batch_get_params(cs_batch b, uint32_t* params3);
+/* times every candidate (R in {32, 64, 128}, RT in {4, 8}, G in {2, 4, 8} x CUs) on this model and keeps the fastest;
+ * cache_path (or NULL): a text file of "key R RT G" lines keyed by device and sizes, read first and appended to */
+int cs_batch_autotune(cs_batch b, int reps, const char* cache_path, uint32_t* best3);
+int cs_batch_run(cs_batch b);
+int cs_batch_wait(cs_batch b);
+int cs_batch_read(cs_batch b, const char* name, void* host, size_t bytes);
+int cs_batch_time(cs_batch b, int iters, float* ms_per_run);
+/* `iters` back-to-back runs with a device event after each: ms[i] = the i-th run's time (for percentiles) */
+int cs_batch_time_each(cs_batch b, int iters, float* ms);
+int cs_batch_stats(cs_batch b, uint64_t* stats4);
+int cs_batch_destroy(cs_batch b);
+
 #ifdef __cplusplus
 }
 #endif

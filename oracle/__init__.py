@@ -28,6 +28,8 @@ def lib():
             "or_mix32": (C.c_uint32, [C.c_uint32]),
             "or_hash_unit": (C.c_float, [C.c_uint32, _S]),
             "or_fill_hash_f32": (None, [C.c_uint32, _P, _S, _S, _S]),
+            "or_fill_hash_f32_base": (None, [C.c_uint32, _P, _S, _S, _S, _S]),
+            "or_strip_partials_f32": (None, [_S, _S, _S, _P, _P, _P, _P, _P, _P, _P]),
             "or_array_seed": (C.c_uint32, [C.c_uint32, C.c_uint32]),
             "or_relative_l2_f32": (C.c_double, [_P, _P, _S]),
             "or_relative_l2_f64": (C.c_double, [_P, _P, _S]),

@@ -59,6 +59,28 @@ void or_fill_hash_f32(uint32_t seed, float* out, uint64_t rows, uint64_t cols, u
         for (uint64_t j = 0; j < cols; ++j) out[(uint64_t)i * ld + j] = or_hash_unit(seed, (uint64_t)i * cols + j);
 }
 
+/* the same with the flat index starting at `base` (instance k of a batch: base = sum of the earlier instances' sizes) */
+void or_fill_hash_f32_base(uint32_t seed, float* out, uint64_t rows, uint64_t cols, uint64_t ld, uint64_t base) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)rows; ++i)
+        for (uint64_t j = 0; j < cols; ++j) out[(uint64_t)i * ld + j] = or_hash_unit(seed, base + (uint64_t)i * cols + j);
+}
+
+/* Pass 1 of the two-pass GEMVER for any n (the last strip may be shorter): P[c][j] = sum over the strip's rows of
+ * fma(rank-2 updated a_ij, y_i, acc), rows ascending; the order of components.hpp:85-97 + :142-155. */
+void or_strip_partials_f32(uint64_t n, uint64_t lda, uint64_t R, const float* A, const float* u1, const float* v1, const float* u2,
+                           const float* v2, const float* y, float* P) {
+    const uint64_t strips = (n + R - 1) / R;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)strips; ++c) {
+        float* p = P + (uint64_t)c * n;
+        for (uint64_t j = 0; j < n; ++j) p[j] = 0.0f;
+        const uint64_t i1 = ((uint64_t)c + 1) * R < n ? ((uint64_t)c + 1) * R : n;
+        for (uint64_t i = (uint64_t)c * R; i < i1; ++i)
+            for (uint64_t j = 0; j < n; ++j) p[j] = fmaf(fmaf(u2[i], v2[j], fmaf(u1[i], v1[j], A[i * lda + j])), y[i], p[j]);
+    }
+}
+
 /* ---- comparison: chain_data.hpp:147-161 -------------------------------------------------------------------------- */
 static double rel_l2_core(const double* num_den) {
     const double num = num_den[0], den = num_den[1];

@@ -24,6 +24,11 @@ uint32_t or_mix32(uint32_t x);
 float    or_hash_unit(uint32_t seed, uint64_t index);
 /* rows x cols row-major with leading dimension ld (elements); element (i, j) = hash_unit(seed, i * cols + j) */
 void or_fill_hash_f32(uint32_t seed, float* out, uint64_t rows, uint64_t cols, uint64_t ld);
+/* the same with the flat index starting at `base` */
+void or_fill_hash_f32_base(uint32_t seed, float* out, uint64_t rows, uint64_t cols, uint64_t ld, uint64_t base);
+/* pass 1 of the two-pass GEMVER for any n: strips of R rows (the last may be shorter), P is ceil(n/R) x n */
+void or_strip_partials_f32(uint64_t n, uint64_t lda, uint64_t R, const float* A, const float* u1, const float* v1, const float* u2,
+                           const float* v2, const float* y, float* P);
 /* seed of array `id` (1 = A .. 7 = z, chain_data.hpp:32) for a base seed */
 uint32_t or_array_seed(uint32_t base_seed, uint32_t id);
 

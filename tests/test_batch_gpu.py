@@ -1,0 +1,183 @@
+"""The batched ragged GEMVER on the device: the offsets scan bit for bit, every instance's x and w against the oracle, the
+persistent launch under graph replay, the autotuner."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from cospanmdao_amd import check, workloads
+
+pytestmark = pytest.mark.gpu
+
+ALPHA, BETA = 1.5, 1.2
+TOL_L2 = 1e-5
+
+
+class Batch:
+    def __init__(self, count, lo, hi, seed=0):
+        self.h = C.c_void_p()
+        check(workloads.cs_batch_create(C.byref(self.h), count, lo, hi, ALPHA, BETA, seed))
+        self.count, self.seed = count, seed
+        self.sizes = np.empty(count, np.uint32)
+        check(workloads.cs_batch_sizes(self.h, self.sizes.ctypes.data_as(C.POINTER(C.c_uint32)), count))
+
+    def params(self, R, RT, G):
+        check(workloads.cs_batch_set_params(self.h, R, RT, G))
+
+    def run(self):
+        check(workloads.cs_batch_run(self.h))
+        check(workloads.cs_batch_wait(self.h))
+
+    def read(self, name, dtype, count):
+        out = np.empty(count, dtype)
+        check(workloads.cs_batch_read(self.h, name.encode(), out.ctypes.data, out.nbytes))
+        return out
+
+    def stats(self):
+        s = (C.c_uint64 * 4)()
+        check(workloads.cs_batch_stats(self.h, s))
+        return list(s)
+
+    def close(self):
+        if self.h:
+            check(workloads.cs_batch_destroy(self.h))
+            self.h = C.c_void_p()
+
+
+def counts_of(sizes, R, RT):
+    K = sizes.size
+    n = sizes.astype(np.uint64)
+    ld = (n + 3) // 4 * 4
+    strips = (n + R - 1) // R
+    blocks = (n + 1023) // 1024
+    c = np.concatenate([n, n * ld // 4, strips * n, strips * blocks, blocks, (n + RT - 1) // RT])
+    assert c.size == 6 * K
+    return c.astype(np.uint32)
+
+
+def instance_inputs(sizes, k, seed):
+    """Instance k's A and vectors as the model generates them: one flat hash stream per array."""
+    L = O.lib()
+    n = int(sizes[k])
+    a_base = int(np.sum(sizes[:k].astype(np.uint64) ** 2))
+    v_base = int(np.sum(sizes[:k].astype(np.uint64)))
+    A = np.empty((n, n), np.float32)
+    L.or_fill_hash_f32_base(seed * 16 + 1, O.ptr(A), n, n, n, a_base)
+    inp = {"A": A}
+    for i, name in enumerate(("u1", "v1", "u2", "v2", "y", "z")):
+        v = np.empty(n, np.float32)
+        L.or_fill_hash_f32_base(seed * 16 + 2 + i, O.ptr(v), 1, n, n, v_base)
+        inp[name] = v
+    return inp
+
+
+def check_instances(b, which, R):
+    v_off = np.concatenate([[0], np.cumsum(b.sizes.astype(np.uint64))]).astype(np.int64)
+    total = int(v_off[-1])
+    x = b.read("x", np.float32, total)
+    w = b.read("w", np.float32, total)
+    worst = 0.0
+    for k in which:
+        n = int(b.sizes[k])
+        inp = instance_inputs(b.sizes, k, b.seed)
+        gx, gw = x[v_off[k]:v_off[k] + n], w[v_off[k]:v_off[k] + n]
+        xr, wr = O.gemver_f64(inp, ALPHA, BETA)
+        ex, ew = O.rel_l2(gx, xr), O.rel_l2(gw, wr)
+        worst = max(worst, ex, ew)
+        assert ex < TOL_L2 and ew < TOL_L2, (k, n, ex, ew)
+        # x keeps the components' order: bit for bit against the strip partials + column reduction of the oracle
+        strips = (n + R - 1) // R
+        P = np.empty((strips, n), np.float32)
+        O.lib().or_strip_partials_f32(n, n, R, O.ptr(inp["A"]), *[O.ptr(inp[q]) for q in ("u1", "v1", "u2", "v2", "y")], O.ptr(P))
+        xf = np.empty(n, np.float32)
+        O.lib().or_col_reduce_f32(n, strips, O.ptr(P), O.ptr(inp["z"]), BETA, O.ptr(xf))
+        assert np.array_equal(gx, xf), (k, n)
+    return worst
+
+
+@pytest.mark.parametrize("R,RT,G", [(32, 4, 3), (32, 8, 64), (64, 8, 17), (128, 4, 512)])
+def test_small_batch(R, RT, G):
+    print("batch small", R, RT, G, flush=True)
+    b = Batch(24, 16, 700, seed=1)
+    try:
+        sizes = np.empty(24, np.uint32)
+        O.lib().or_ragged_sizes(1, 24, 16, 700, O.ptr(sizes))
+        assert np.array_equal(b.sizes, sizes)
+        b.params(R, RT, G)
+        b.run()
+        counts = b.read("counts", np.uint32, 6 * 24)
+        assert np.array_equal(counts, counts_of(b.sizes, R, RT))
+        offs = b.read("offs", np.uint32, 6 * 24 + 1)
+        assert np.array_equal(offs, O.exclusive_scan_u32(counts))  # the device scan, bit for bit
+        res = b.read("result", np.uint32, 2)
+        assert res[0] == offs[-1] and res[1] == 0
+        check_instances(b, range(24), R)
+    finally:
+        b.close()
+
+
+def test_replays_and_parameter_changes():
+    b = Batch(40, 64, 1200, seed=2)
+    try:
+        b.params(64, 8, 96)
+        for _ in range(4):
+            b.run()
+        st = b.stats()
+        assert st[2] >= 1 and st[3] >= 2, st  # captured once, then replayed
+        check_instances(b, [0, 7, 39], 64)
+        b.params(32, 4, 40)  # new parameters: new counts, new grid; eager, then captured again
+        for _ in range(3):
+            b.run()
+        check_instances(b, [3, 20, 39], 32)
+    finally:
+        b.close()
+
+
+def test_full_batch_512():
+    """512 instances, N log-uniform in [64, 4096], seed 0: the measured configuration (about 4 GB of A)."""
+    print("batch full 512", flush=True)
+    b = Batch(512, 64, 4096, seed=0)
+    try:
+        b.params(64, 8, 1024)
+        b.run()
+        b.run()
+        b.run()  # a replay
+        counts = b.read("counts", np.uint32, 6 * 512)
+        offs = b.read("offs", np.uint32, 6 * 512 + 1)
+        assert np.array_equal(offs, O.exclusive_scan_u32(counts))
+        assert np.array_equal(offs[:513].astype(np.int64), np.concatenate([[0], np.cumsum(b.sizes.astype(np.int64))]))
+        order = np.argsort(b.sizes)
+        which = sorted(set(order[:3].tolist() + order[-3:].tolist() + list(range(0, 512, 64))))
+        worst = check_instances(b, which, 64)
+        print(f"worst rel-L2 over {len(which)} instances: {worst:.2e}", flush=True)
+    finally:
+        b.close()
+
+
+def test_autotune_picks_a_candidate_and_caches(tmp_path):
+    b = Batch(32, 64, 900, seed=3)
+    try:
+        best = (C.c_uint32 * 3)()
+        path = str(tmp_path / "autotune.txt")
+        check(workloads.cs_batch_autotune(b.h, 2, path.encode(), best))
+        assert best[0] in (32, 64, 128) and best[1] in (4, 8) and best[2] > 0
+        assert os.path.exists(path)
+        again = (C.c_uint32 * 3)()
+        check(workloads.cs_batch_autotune(b.h, 2, path.encode(), again))  # from the cache
+        assert list(again) == list(best)
+        b.run()
+        check_instances(b, [0, 31], int(best[0]))
+    finally:
+        b.close()
+
+
+def test_bad_parameters_are_errors():
+    b = Batch(4, 64, 128)
+    try:
+        assert workloads.cs_batch_set_params(b.h, 16, 8, 64) == 1
+        assert workloads.cs_batch_set_params(b.h, 64, 5, 64) == 1
+        assert workloads.cs_batch_set_params(b.h, 64, 8, 0) == 1
+    finally:
+        b.close()

@@ -334,3 +334,33 @@ def test_ragged_sizes_bounds():
     assert s.min() >= 64 and s.max() <= 4096
     logs = np.log(s.astype(np.float64) / 64) / np.log(64)
     assert 0.4 < logs.mean() < 0.6 and logs.min() < 0.05 and logs.max() > 0.95
+
+
+def test_strip_partials_match_components_and_ragged_strips():
+    L = O.lib()
+    n, R = 256, 64
+    inp = O.gemver_inputs(n, base_seed=2)
+    B = np.empty((n, n), np.float32)
+    P = np.empty((n // R, n), np.float32)
+    L.or_rank2_update_f32(n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2")], O.ptr(B))
+    L.or_gemvt_partial_f32(n, R, O.ptr(B), O.ptr(inp["y"]), O.ptr(P))
+    P2 = np.empty_like(P)
+    L.or_strip_partials_f32(n, n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y")], O.ptr(P2))
+    assert np.array_equal(P, P2)
+    # a ragged n: the last strip is shorter; the strip sums equal the FP64 column sums of the updated matrix
+    n = 150
+    inp = O.gemver_inputs(n, base_seed=4)
+    P3 = np.empty(((n + R - 1) // R, n), np.float32)
+    L.or_strip_partials_f32(n, n, R, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y")], O.ptr(P3))
+    d = {k: inp[k].astype(np.float64) for k in ("u1", "v1", "u2", "v2", "y")}
+    Bd = inp["A"].astype(np.float64) + np.outer(d["u1"], d["v1"]) + np.outer(d["u2"], d["v2"])
+    want = (Bd * d["y"][:, None]).sum(axis=0)
+    assert np.allclose(P3.astype(np.float64).sum(axis=0), want, rtol=1e-5, atol=1e-4)
+
+
+def test_hash_with_base_is_the_flat_stream():
+    L = O.lib()
+    full = O.hash_matrix(11, 5, 40)
+    part = np.empty((3, 40), np.float32)
+    L.or_fill_hash_f32_base(11, O.ptr(part), 3, 40, 40, 2 * 40)
+    assert np.array_equal(part, full[2:])
