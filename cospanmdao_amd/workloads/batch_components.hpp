@@ -165,11 +165,15 @@ public:
         const std::uint32_t tid      = static_cast<std::uint32_t>(it.get_local_id(0));
         __shared__ std::uint32_t s_unit;
 
-        for (;;) {
+        // a work-group takes each unit at most once, so T + 1 fetches bound the loop whatever the state holds
+        for (std::uint32_t fetches = 0; fetches <= T; ++fetches) {
+            // The barrier BEFORE the leader's fetch keeps the leader-only blocks of two iterations (the previous unit's
+            // completion and this fetch) apart: with nothing between them the compiler merges them across the back edge
+            // into a nested loop whose divergent exit lets the other lanes re-read the previous unit forever.
+            __syncthreads();
             if (tid == 0) s_unit = __hip_atomic_fetch_add(&st[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __syncthreads();
             const std::uint32_t u = s_unit;
-            __syncthreads();
             if (u >= T) break;
             if (u < T1) {
                 const std::uint32_t k = cols.find(3, u);
