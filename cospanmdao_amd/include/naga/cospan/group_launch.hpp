@@ -15,6 +15,7 @@
 #include <cstddef>
 #include <cstring>
 #include <functional>
+#include <new>
 #include <type_traits>
 #include <cstdint>
 #include <stdexcept>
@@ -123,6 +124,20 @@ struct LaunchRecord {
         put(args.data(), args.size());
     }
 };
+
+/// Member-wise copies of a stage's arguments (the padding of the destination is left as it is).
+template <class X>
+inline void assign_stage(StageArgs<X>& d, const StageArgs<X>& s) {
+    for (int i = 0; i < 3; ++i) d.units[i] = s.units[i];
+    d.in  = s.in;
+    d.out = s.out;
+    if constexpr (!std::is_empty_v<local_mem_t<X>>) d.local = s.local;
+}
+template <class H, class... T>
+inline void assign_pack(ArgPack<H, T...>& d, const ArgPack<H, T...>& s) {
+    assign_stage(d.head, s.head);
+    if constexpr (sizeof...(T) > 0) assign_pack(d.tail, s.tail);
+}
 
 /// Builds the geometry of a group and launches its kernel.
 template <class... X>
@@ -260,8 +275,12 @@ public:
         rec.block[1] = wg[1];
         rec.block[2] = wg[0];
         rec.lds      = static_cast<std::size_t>(lds);
-        rec.args.resize(sizeof ga);
-        std::memcpy(rec.args.data(), &ga, sizeof ga);
+        // the argument bytes: a zeroed buffer, then member-wise assignment, so padding is zero and identical launches have
+        // identical bytes (the graph-replay signature compares them)
+        rec.args.assign(sizeof ga, std::byte{0});
+        auto* out = ::new (static_cast<void*>(rec.args.data())) GroupArgs<X...>;
+        out->geo  = ga.geo;
+        assign_pack(out->args, ga.args);
         return rec;
     }
 

@@ -290,6 +290,7 @@ struct ModelMeta {
             }
         }
         keep->has_producer = keep->has_producer || gone->has_producer;
+        keep->scratch      = keep->scratch || gone->scratch;
         if (!keep->init && gone->init) {
             keep->init    = gone->init;
             keep->pending = gone->pending;

