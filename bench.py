@@ -212,6 +212,19 @@ def cpu_baseline(n, R, budget_s=12.0):
     }
 
 
+def copy_calibration(gib=1):
+    """A plain device-to-device copy of `gib` GiB, three times (torch's copy kernel): the reference point that scales
+    rocprofv3's FETCH_SIZE and WRITE_SIZE counters to bytes, since each copy reads and writes exactly gib GiB."""
+    import torch
+
+    x = torch.empty(gib * 2**28, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -223,12 +236,16 @@ def main():
     ap.add_argument("--style", default="twopass", choices=["twopass", "twopass_1"])
     ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true", help="only the 1 GiB copy that scales the HBM counters, then exit")
     ap.add_argument("--batch", action="store_true",
                     help="also time the batched ragged GEMVER (off by default until its persistent kernel is verified on a GPU)")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (how the path would shard is "
                  "described in DESIGN.md)")
+    if a.copy_calibration:
+        copy_calibration()
+        return
     from cospanmdao_amd import check, hip
 
     check(hip.cs_hip_init(0))
