@@ -68,7 +68,7 @@ class Gemver:
 
 
 class Ns:
-    def __init__(self, style, n, store=0, ty=16):
+    def __init__(self, style, n, store=0, ty=32):
         from cospanmdao_amd import check, workloads
 
         self.w, self.check = workloads, check
@@ -137,7 +137,7 @@ def bench_precision(extra, steps):
     n = 32768
     rec = {}
     for store, name, elem in ((0, "f32", 4), (1, "bf16", 2), (2, "e4m3", 1)):
-        g = Gemver(TWOPASS, n, 128, store=store)
+        g = Gemver(TWOPASS, n, 256, store=store)
         ms = timed_steps(g, steps, 3)
         ln = g.launches(3)
         g.close()
@@ -231,8 +231,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--R", type=int, default=128)
-    ap.add_argument("--rt2", type=int, default=8)
+    ap.add_argument("--R", type=int, default=256)
+    ap.add_argument("--rt2", type=int, default=4)
     ap.add_argument("--style", default="twopass", choices=["twopass", "twopass_1"])
     ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")

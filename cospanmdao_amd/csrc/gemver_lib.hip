@@ -27,7 +27,7 @@ struct cs_gemver_s {
     std::size_t                n = 0, R = 64;
     float                      alpha = 1.0f, beta = 1.0f;
     std::uint32_t              seed = 0;
-    int                        rt2  = 8;
+    int                        rt2  = 4;
     std::unique_ptr<cs::Model> model;
 };
 

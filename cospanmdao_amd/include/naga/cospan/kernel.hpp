@@ -192,7 +192,9 @@ __device__ inline LogicalGroup locate_physical(std::uint32_t split, const Segmen
 inline constexpr std::size_t kTicketHeader = 2;
 CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return kTicketHeader + segments + total; }
 
-__device__ inline std::uint32_t acquire_load(std::uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
+/// A spin-wait load: relaxed at agent scope (coherent across the XCDs' L2s without invalidating them on every poll); the
+/// waits below order what follows with ONE acquire fence once the condition holds.
+__device__ inline std::uint32_t poll_load(std::uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 /// Waits only ever target groups with earlier tickets, which have started and finish without waiting on later ones, so
 /// they end; the bound (about a second) only guarantees that corrupted ticket state cannot hang the device.
@@ -203,7 +205,7 @@ __device__ inline void ticket_wait_group(std::uint32_t* ts, std::uint32_t segmen
                                          const std::uint64_t* c) {
     if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
         const std::uint64_t t = tbl[s].first + segment_linear(tbl[s], c);
-        for (std::uint32_t spin = 0; acquire_load(&ts[kTicketHeader + segments + t]) == 0 && spin < kSpinLimit; ++spin)
+        for (std::uint32_t spin = 0; poll_load(&ts[kTicketHeader + segments + t]) == 0 && spin < kSpinLimit; ++spin)
             __builtin_amdgcn_s_sleep(1);
     }
     __syncthreads();
@@ -215,7 +217,7 @@ __device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo
     if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
         for (std::uint32_t i = 0; i < upto; ++i)
             for (std::uint32_t spin = 0;
-                 static_cast<std::uint64_t>(acquire_load(&ts[kTicketHeader + i])) != tbl[i].count && spin < kSpinLimit; ++spin)
+                 static_cast<std::uint64_t>(poll_load(&ts[kTicketHeader + i])) != tbl[i].count && spin < kSpinLimit; ++spin)
                 __builtin_amdgcn_s_sleep(1);
     }
     __syncthreads();

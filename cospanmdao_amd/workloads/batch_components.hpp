@@ -218,7 +218,7 @@ public:
     /// cannot hang the device), then the group.
     __device__ static void wait_for(std::uint32_t* counter, std::uint32_t target, std::uint32_t tid) {
         if (tid == 0)
-            for (std::uint32_t spin = 0; __hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target && spin < (1u << 24); ++spin)
+            for (std::uint32_t spin = 0; cs::detail::poll_load(counter) < target && spin < (1u << 24); ++spin)
                 __builtin_amdgcn_s_sleep(1);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");

@@ -160,11 +160,14 @@ public:
         const std::uint32_t n = u32(in.P.shape()[1]), strips = u32(in.P.shape()[0]);
         const float         scale  = in.scale[std::size_t{0}];
         const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        const float*        P      = cs::raw::data(in.P);
+        const std::size_t   ld     = in.P.stride()[0];
         for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
-            float acc = 0.0f;
+            float        acc = 0.0f;
+            const float* pj  = P + j;
             // the sum stays in ascending c; unrolling keeps 16 independent loads in flight per work-item
 #pragma unroll 16
-            for (std::uint32_t c = 0; c < strips; ++c) acc += in.P[cs::id<2>{c, j}];
+            for (std::uint32_t c = 0; c < strips; ++c) acc += pj[static_cast<std::size_t>(c) * ld];
             if constexpr (Z)
                 out.x[cs::id<1>{j}] = fmaf(scale, acc, in.z[cs::id<1>{j}]);
             else
@@ -324,9 +327,14 @@ class AhatTPartial : public cs::Component<AhatTPartial<S>> {
 public:
     static constexpr int max_work_group_size = kWg;
     using St                                 = Storage<S>;
-    static constexpr int kVec                = St::kVec;
+    /// elements of one 16-byte load
+    static constexpr int kEl = St::kVec;
+    /// 16-byte chunks per work-item: 16 columns per work-item whatever the storage
+    static constexpr int kV = 16 / kEl;
+    /// columns of one chunk row across the work-group (one load instruction covers 64 * kEl contiguous columns per wave)
+    static constexpr int kChunk = kWg * kEl;
     /// columns of one work-group
-    static constexpr int kCols = kWg * kVec;
+    static constexpr int kCols = kChunk * kV;
 
     struct Inputs {
         cs::Input<S, 2>          A;
@@ -338,56 +346,84 @@ public:
         cs::Output<DType::f32, 2> P;
     };
 
-    /// group (c, cb): strip c of R rows, column block cb of kCols columns
+    /// group (c, cb): strip c of R rows, column block cb of kCols columns; work-item t owns the kV chunks of kEl columns
+    /// at cb * kCols + q * kChunk + t * kEl (q < kV), interleaved so that every load instruction of a wave reads one
+    /// contiguous run. Each column's sum over the strip is one fma chain in k order (the components' order).
     __device__ static void compute_forward(cs::GroupItem<2> it, const Inputs& in, const Outputs& out) {
         const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
         const std::size_t   lda  = in.A.stride()[0];
         const std::uint32_t R    = in.R[std::size_t{0}];
         const std::uint32_t c = u32(it.get_group(0)), cb = u32(it.get_group(1));
-        const std::uint32_t j0 = (cb * kWg + u32(it.get_local_id(1))) * kVec;
-        if (j0 >= n) return;
+        const std::uint32_t jb = cb * kCols + u32(it.get_local_id(1)) * kEl;
+        if (jb >= n) return;
         const std::uint32_t i0 = c * R, i1 = (i0 + R < rows) ? i0 + R : rows;
         const auto*         A  = cs::raw::data(in.A);
         const float*        u1 = cs::raw::data(in.u1);
         const float*        u2 = cs::raw::data(in.u2);
         const float*        y  = cs::raw::data(in.y);
         const float*        sc = cs::raw::data(in.row_scale);
-        float               acc[kVec];
-#pragma unroll
-        for (int v = 0; v < kVec; ++v) acc[v] = 0.0f;
+        const float*        V1 = cs::raw::data(in.v1);
+        const float*        V2 = cs::raw::data(in.v2);
 
-        if (j0 + kVec <= n) {
-            float v1[kVec], v2[kVec];
-            load_f32<kVec>(cs::raw::data(in.v1) + j0, v1);
-            load_f32<kVec>(cs::raw::data(in.v2) + j0, v2);
-            const typename St::T* a = A + static_cast<std::size_t>(i0) * lda + j0;
-#pragma unroll 8
-            for (std::uint32_t i = i0; i < i1; ++i, a += lda) {
-                const typename St::Raw r = load_stream<typename St::Raw>(a);
-                float                  av[kVec];
-                St::widen(r, S == DType::e4m3 ? sc[i] : 1.0f, av);
-                const float yi = y[i], a1 = u1[i], a2 = u2[i];
+        // chunks wholly inside the row take the vector path; at most one chunk per work-item is partial
+        bool          full[kV];
+        std::uint32_t part_q = kV, part_m = 0;
+        float         acc[kV][kEl], v1[kV][kEl], v2[kV][kEl];
 #pragma unroll
-                for (int v = 0; v < kVec; ++v) acc[v] = fmaf(rank2(av[v], a1, v1[v], a2, v2[v]), yi, acc[v]);
+        for (int q = 0; q < kV; ++q) {
+            const std::uint32_t j = jb + static_cast<std::uint32_t>(q) * kChunk;
+            full[q]               = j + kEl <= n;
+            if (!full[q] && j < n) {
+                part_q = static_cast<std::uint32_t>(q);
+                part_m = n - j;
             }
-            float* p = &out.P[cs::id<2>{c, j0}];
 #pragma unroll
-            for (int q = 0; q < kVec / 4; ++q) {
-                using F4                        = float __attribute__((ext_vector_type(4)));
-                reinterpret_cast<F4*>(p)[q]     = F4{acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]};
+            for (int e = 0; e < kEl; ++e) acc[q][e] = 0.0f;
+            if (full[q]) {
+                load_f32<kEl>(V1 + j, v1[q]);
+                load_f32<kEl>(V2 + j, v2[q]);
             }
-        } else {
-            // the ragged tail of a row: element by element, the same order
-            const std::uint32_t m = n - j0;
+        }
+        const typename St::T* a = A + static_cast<std::size_t>(i0) * lda + jb;
+#pragma unroll 4
+        for (std::uint32_t i = i0; i < i1; ++i, a += lda) {
+            typename St::Raw r[kV];
+#pragma unroll
+            for (int q = 0; q < kV; ++q)
+                if (full[q]) r[q] = load_stream<typename St::Raw>(a + q * kChunk);
+            const float s = S == DType::e4m3 ? sc[i] : 1.0f, yi = y[i], a1 = u1[i], a2 = u2[i];
+#pragma unroll
+            for (int q = 0; q < kV; ++q) {
+                if (!full[q]) continue;
+                float av[kEl];
+                St::widen(r[q], s, av);
+#pragma unroll
+                for (int e = 0; e < kEl; ++e) acc[q][e] = fmaf(rank2(av[e], a1, v1[q][e], a2, v2[q][e]), yi, acc[q][e]);
+            }
+        }
+        float* p = &out.P[cs::id<2>{c, 0}];
+#pragma unroll
+        for (int q = 0; q < kV; ++q) {
+            if (!full[q]) continue;
+            using F4 = float __attribute__((ext_vector_type(4)));
+            float* pq = p + jb + q * kChunk;
+#pragma unroll
+            for (int k4 = 0; k4 < kEl / 4; ++k4)
+                reinterpret_cast<F4*>(pq)[k4] = F4{acc[q][4 * k4], acc[q][4 * k4 + 1], acc[q][4 * k4 + 2], acc[q][4 * k4 + 3]};
+        }
+        if (part_q < kV) {
+            // the ragged end of the row: element by element, the same order
+            const std::uint32_t j = jb + part_q * kChunk;
+            float               pacc[kEl];
+            for (std::uint32_t e = 0; e < part_m; ++e) pacc[e] = 0.0f;
             for (std::uint32_t i = i0; i < i1; ++i) {
-                const float s  = S == DType::e4m3 ? sc[i] : 1.0f;
-                const float yi = y[i], a1 = u1[i], a2 = u2[i];
-                for (std::uint32_t v = 0; v < m; ++v) {
-                    const float av = St::widen1(A[static_cast<std::size_t>(i) * lda + j0 + v], s);
-                    acc[v]         = fmaf(rank2(av, a1, in.v1[cs::id<1>{j0 + v}], a2, in.v2[cs::id<1>{j0 + v}]), yi, acc[v]);
+                const float s = S == DType::e4m3 ? sc[i] : 1.0f, yi = y[i], a1 = u1[i], a2 = u2[i];
+                for (std::uint32_t e = 0; e < part_m; ++e) {
+                    const float av = St::widen1(A[static_cast<std::size_t>(i) * lda + j + e], s);
+                    pacc[e]        = fmaf(rank2(av, a1, V1[j + e], a2, V2[j + e]), yi, pacc[e]);
                 }
             }
-            for (std::uint32_t v = 0; v < m; ++v) out.P[cs::id<2>{c, j0 + v}] = acc[v];
+            for (std::uint32_t e = 0; e < part_m; ++e) p[j + e] = pacc[e];
         }
     }
 

@@ -146,7 +146,8 @@ public:
         const std::uint32_t jl = j0 == 0 ? nx - 1 : j0 - 1, jr = j0 + kVec >= nx ? 0 : j0 + kVec;
 
         auto row = [&](const typename F::T* f, std::uint32_t i, float (&a)[kVec]) {
-            F::widen(__builtin_nontemporal_load(reinterpret_cast<const typename F::Raw*>(f + static_cast<std::size_t>(i) * ld + j0)), a);
+            // cached loads: the halo rows a tile shares with its neighbours are then served by the L2
+            F::widen(*reinterpret_cast<const typename F::Raw*>(f + static_cast<std::size_t>(i) * ld + j0), a);
         };
         const std::uint32_t im = i0 == 0 ? ny - 1 : i0 - 1;
         float um[kVec], uc[kVec], up[kVec], vm[kVec], vc[kVec], vp[kVec], pm[kVec], pc[kVec], pp[kVec];
