@@ -7,7 +7,7 @@ replays one captured graph. `value` = algorithmic HBM bytes per step (2 N^2 * 4 
 time per step, in GB/s.
 
 Beside it (`extra`): the same GEMVER as the reference's four unfused components (B materialised); the NS residual at
-4096^2 fused (one launch) and unfused; the batched ragged GEMVER (512 instances, N log-uniform in [64, 4096]: instances/s,
+4096^2 fused (one fused group) and unfused; the batched ragged GEMVER (512 instances, N log-uniform in [64, 4096]: instances/s,
 p50/p99 per replay, per-byte rate against the fused N=16384 rate); GEMVER N = 32768 with A stored f32, bf16 and e4m3 +
 row scale (speedups) and the NS residual at 8192^2 with bf16 fields. Errors against the FP64 oracle are measured by the
 GPU tests (tests/test_*_gpu.py), not here. The roofline object is for the dominant kernel of the headline workload; the
@@ -68,7 +68,7 @@ class Gemver:
 
 
 class Ns:
-    def __init__(self, style, n, store=0, ty=32):
+    def __init__(self, style, n, store=0, ty=64):
         from cospanmdao_amd import check, workloads
 
         self.w, self.check = workloads, check
@@ -137,7 +137,7 @@ def bench_precision(extra, steps):
     n = 32768
     rec = {}
     for store, name, elem in ((0, "f32", 4), (1, "bf16", 2), (2, "e4m3", 1)):
-        g = Gemver(TWOPASS, n, 256, store=store)
+        g = Gemver(TWOPASS, n, 128, store=store)
         ms = timed_steps(g, steps, 3)
         ln = g.launches(3)
         g.close()
@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=16384)
-    ap.add_argument("--R", type=int, default=256)
+    ap.add_argument("--R", type=int, default=128)
     ap.add_argument("--rt2", type=int, default=4)
     ap.add_argument("--style", default="twopass", choices=["twopass", "twopass_1"])
     ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
@@ -267,7 +267,7 @@ def main():
         kname = "gemver pass 1 (AhatTPartial<f32>)" if k == 0 else f"gemver pass 2 (AhatGemv<f32,{a.rt2}>)"
         kms = launch_ms[k]
     else:
-        kbytes, kname, kms = fused_bytes, "gemver fused two-pass (one launch)", launch_ms[0]
+        kbytes, kname, kms = fused_bytes, "gemver fused two-pass (one fused group)", launch_ms[0]
     achieved = gbs(kbytes, kms)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
                 "traffic": None, "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}

@@ -1,15 +1,20 @@
 #pragma once
 
 // =============================================================================
-// naga/cospan/fused_component.hpp — fuse_components, segment, global_barrier: ONE launch per fused group
+// naga/cospan/fused_component.hpp — fuse_components, segment, global_barrier: one kernel per fused group
 // =============================================================================
 //
 // The reference's helpers (cospanmdao/include/naga/cospan/fused_component.hpp:455-569) with the same meaning:
-//   cs::fuse_components<opts>(name, [connects_desc(...)], items...)   a leaf whose stages run in ONE launch
+//   cs::fuse_components<opts>(name, [connects_desc(...)], items...)   a leaf whose stages run in one kernel
 //   cs::segment<cs::local_mem::shared|partitioned, cs::range_mode::whole|own>(descs...)
 //   cs::global_barrier()                                               every group of the next segment waits for every
 //                                                                      group of the earlier segments
 //   options: cs::group_waits_enabled, cs::range_split::{flatten_and_split, pad_and_split, padded}
+// How a global barrier is kept: with group waits enabled the whole group runs as ONE launch in ticket order and a group
+// of a later segment waits on the earlier segments' counters. Without group waits each barrier-free run of segments is
+// one launch of the same kernel, in order on one stream: the launch boundary is the barrier. On CDNA the second form is
+// the fast one, because a waiting protocol needs every group to make its writes visible at agent scope (an L2
+// write-back per group) where a launch boundary does it once.
 // A stage outside a segment is a segment of its own. The group's stage types are a compile-time list (the item types),
 // so its kernel is detail::group_kernel<MaxThreads, X...>, instantiated by hipcc in the translation unit that builds the
 // model: the compile-time counterpart of the reference's JIT-assembled indexed call sequence (kernel.hpp:911-1004).
@@ -94,8 +99,7 @@ template <bool Waits, Split Sp, class... Items>
 class FusedGroup : public FusedBase {
     using Stages = typename concat<typename item_stages<Items>::type...>::type;
     using L      = launcher_of<Stages>;
-    static constexpr int  N        = L::N;
-    static constexpr bool kBarrier = (is_boundary<Items>::value || ...);
+    static constexpr int N = L::N;
 
 public:
     explicit FusedGroup(ConnectsDesc c, Items... items) : connects_{std::move(c)}, items_{std::move(items)...} {}
@@ -122,7 +126,8 @@ public:
         StagePlan           plans[N];
         typename L::pack    args{};
         prepare_stages(m, plans, args, std::make_index_sequence<N>{});
-        return launcher_.prepare(Sp, Waits || kBarrier, segs_, plans, args, m.exec.stream(0), path_);
+        // global barriers without group waits become launch boundaries (GroupLauncher::prepare)
+        return launcher_.prepare(Sp, Waits, segs_, plans, args, m.exec.stream(0), path_);
     }
 
 private:

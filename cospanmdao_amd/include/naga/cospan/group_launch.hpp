@@ -7,8 +7,9 @@
 // The host half of the reference's GroupLauncher::launch (cospanmdao/include/naga/cospan/component.hpp:779-970): from the
 // stages' launch facts (units, work-group shape, LDS bytes) and the segment layout it builds the geometry (one entry per
 // stage, the segment table), the physical grid of the range_split strategy (component.hpp:936-954), the LDS block
-// (partitioned: a slice per stage; shared: one block), the ticket state when the group waits, and submits ONE kernel
-// through the C boundary (cs_hip_launch). Checks with the reference's messages: one work-group shape per group
+// (partitioned: a slice per stage; shared: one block), the ticket state when the group waits, and submits the group's
+// kernel through the C boundary (cs_hip_launch): once, or once per barrier-free run of segments when the group's global
+// barriers are launch boundaries. Checks with the reference's messages: one work-group shape per group
 // (component.hpp:815-829), the LDS limit (:880-890).
 
 #include <algorithm>
@@ -100,16 +101,20 @@ struct LaunchRecord {
     std::string            name;
     /// A custom-op launch (CustomComponent): native work enqueued on the stream instead of a group kernel.
     std::function<void(cs_hip_stream)> custom;
+    /// Launches that follow this one on the same stream (a fused group whose global barriers are launch boundaries).
+    std::vector<LaunchRecord> then;
 
-    bool empty() const { return kernel == nullptr && !custom; }
+    bool empty() const { return kernel == nullptr && !custom && then.empty(); }
     void submit(cs_hip_stream s) const {
         if (custom) {
             custom(s);
             return;
         }
-        if (empty()) return;
-        void* a[] = {const_cast<std::byte*>(args.data())};
-        hip_check(cs_hip_launch(kernel, grid, block, lds, a, s), "launch of '" + name + "'");
+        if (kernel) {
+            void* a[] = {const_cast<std::byte*>(args.data())};
+            hip_check(cs_hip_launch(kernel, grid, block, lds, a, s), "launch of '" + name + "'");
+        }
+        for (const LaunchRecord& t : then) t.submit(s);
     }
     /// Appends the identity of this launch to `sig`.
     void append_signature(std::vector<std::byte>& sig) const {
@@ -122,6 +127,7 @@ struct LaunchRecord {
         put(block, sizeof block);
         put(&lds, sizeof lds);
         put(args.data(), args.size());
+        for (const LaunchRecord& t : then) t.append_signature(sig);
     }
 };
 
@@ -167,17 +173,47 @@ public:
     }
 
     /// The launch of the group as a record (nothing is submitted; the ticket state is allocated on `s` when first needed).
+    /// `waits`: group waits (ticket mode for the whole group). Without them, every global barrier is a launch boundary: each
+    /// barrier-free run of segments is one launch of the same kernel, in order on one stream, so no group ever waits and
+    /// no group pays an agent-scope release to tell others it is done (on CDNA that release writes the L2 back).
     LaunchRecord prepare(Split split, bool waits, const std::vector<SegmentPlan>& segs, const StagePlan (&stages)[N],
                          const ArgPack<X...>& args, cs_hip_stream s, const std::string& group_name) {
+        const std::size_t nseg = segs.size();
+        if (nseg == 0 || nseg > static_cast<std::size_t>(N)) throw std::logic_error("naga::cospan: '" + group_name + "': bad segment layout");
+        bool barrier = false;
+        for (const SegmentPlan& sp : segs) barrier = barrier || sp.boundary == Boundary::global_barrier;
+        if (waits || !barrier) return prepare_run(split, waits, segs, 0, nseg, stages, args, s, group_name);
+        LaunchRecord rec;
+        rec.name       = group_name;
+        bool        first = true;
+        std::size_t a     = 0;
+        for (std::size_t b = 1; b <= nseg; ++b) {
+            if (b < nseg && segs[b].boundary != Boundary::global_barrier) continue;
+            LaunchRecord part = prepare_run(split, false, segs, a, b, stages, args, s, group_name);
+            if (first)
+                rec = std::move(part);
+            else
+                rec.then.push_back(std::move(part));
+            first = false;
+            a     = b;
+        }
+        return rec;
+    }
+
+private:
+    /// One launch running segments [a, b) of the group (their indices renumbered from 0; the other stages never run).
+    LaunchRecord prepare_run(Split split, bool waits, const std::vector<SegmentPlan>& all, std::size_t a, std::size_t b,
+                             const StagePlan (&stages)[N], const ArgPack<X...>& args, cs_hip_stream s, const std::string& group_name) {
         LaunchRecord    rec;
         rec.name = group_name;
         GroupArgs<X...> ga{};
         ga.args                 = args;
         Geometry<N>&  geo       = ga.geo;
+        const std::vector<SegmentPlan> segs(all.begin() + static_cast<std::ptrdiff_t>(a), all.begin() + static_cast<std::ptrdiff_t>(b));
         const auto    nseg      = static_cast<std::uint32_t>(segs.size());
-        if (nseg == 0 || nseg > static_cast<std::uint32_t>(N)) throw std::logic_error("naga::cospan: '" + group_name + "': bad segment layout");
         geo.segments = nseg;
         geo.split    = static_cast<std::uint32_t>(split);
+        for (int k = 0; k < N; ++k) geo.stage[k].segment = kNoSegment;
 
         // the one work-group shape of the group (group-range stages)
         std::uint32_t wg[3] = {1, 1, 1};
@@ -284,7 +320,6 @@ public:
         return rec;
     }
 
-private:
     TicketState tickets_;
 };
 

@@ -190,6 +190,8 @@ __device__ inline LogicalGroup locate_physical(std::uint32_t split, const Segmen
 // ---- ticket state ---------------------------------------------------------------------------------------------------
 // [0] ticket counter, [1] finished counter, [2, 2+S) per-segment finished counts, [2+S, 2+S+T) per-ticket done flags.
 inline constexpr std::size_t kTicketHeader = 2;
+/// The segment of a stage that does not run in a launch (a fused group split at its global barriers).
+inline constexpr std::uint32_t kNoSegment = 0xffffffffu;
 CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return kTicketHeader + segments + total; }
 
 /// A spin-wait load: relaxed at agent scope (coherent across the XCDs' L2s without invalidating them on every poll); the

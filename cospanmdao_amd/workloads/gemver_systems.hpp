@@ -10,9 +10,9 @@
 //
 //   unfused    Rank2Update, GemvTPartial, ColReduce, Gemv: four launches (reference systems.hpp:112-122)
 //   fused      fuse_components(connects, segment<range_mode::whole>(r2u, gtp)), cred, gemv: three launches (:105-111)
-//   naive      fuse_components(segment<whole>(r2u, gtp), global_barrier(), cred, global_barrier(), gemv): one launch (:82-104)
+//   naive      fuse_components(segment<whole>(r2u, gtp), global_barrier(), cred, global_barrier(), gemv): one fused group (:82-104)
 //   twopass    AhatTPartial, ColReduce, AhatGemv: Â never stored, A read twice: three launches
-//   twopass_1  the same three stages with global barriers: one launch
+//   twopass_1  the same three stages as one fused group with global barriers
 //
 // The reference ties one `scale` into cred and gemv (alpha = beta = 1/sqrt(n), systems.hpp:79,136-137); this System takes
 // the two GEMVER scalars separately (x = beta Â^T y + z, w = alpha Â x).

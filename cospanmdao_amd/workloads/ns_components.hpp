@@ -13,7 +13,8 @@
 // The physics is this build's (the reference's ns-residual is compressible, examples/ns-residual/physics.hpp); the
 // STRUCTURE follows the reference's fused example (examples/ns-residual/systems.hpp:88-140, README.md:40-57): a tile
 // stage writing the residual and a per-tile partial of the diagnostics, a global_barrier, then a deterministic reduction
-// of the partials in a fixed order (tile.hpp:212-229), all in ONE launch; and an unfused variant of separate kernels with
+// of the partials in a fixed order (tile.hpp:212-229), all in one fused group (its barrier is a launch boundary:
+// the stencil, then the reduction, as two launches of one kernel); and an unfused variant of separate kernels with
 // intermediates in global memory (components.hpp:636-852).
 //
 // NsTile<S>: a work-group owns TY rows x (256 * V) columns; each work-item owns V consecutive columns (one 16-byte load

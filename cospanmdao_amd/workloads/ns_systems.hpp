@@ -1,13 +1,13 @@
 #pragma once
 
 // =============================================================================
-// cospanmdao_amd/workloads/ns_systems.hpp — the NS residual as a System: fused (one launch) or unfused
+// cospanmdao_amd/workloads/ns_systems.hpp — the NS residual as a System: fused (one fused group) or unfused
 // =============================================================================
 //
 // Ports: inputs u v p (the stored fields), coef ({1 / 2h, 1 / h^2, nu}); outputs Ru Rv Rc, sq (FP64 sums of squares),
 // norm (FP64 L2 norms of Ru, Rv, Rc and of all three).
 //
-//   fused    fuse_components("ns", connects, NsTile<S>, global_barrier(), NsNorm): ONE launch (the structure of the
+//   fused    fuse_components("ns", connects, NsTile<S>, global_barrier(), NsNorm): one fused group (the structure of the
 //            reference's fused ns-residual: tile stage, barrier, deterministic reduction; examples/ns-residual/systems.hpp:88-140)
 //   unfused  NsGradients -> NsCombine -> NsSquares -> NsNorm: four launches, derivatives through global memory
 

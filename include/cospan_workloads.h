@@ -23,9 +23,9 @@ extern "C" {
  * Styles (the first three are the reference's, systems.hpp:33-48):
  *   CS_GEMVER_UNFUSED   four components, four launches (B materialised)
  *   CS_GEMVER_FUSED     fuse_components(segment<range_mode::whole>(r2u, gtp)), then cred, gemv: three launches
- *   CS_GEMVER_NAIVE     one launch: segment<whole>(r2u, gtp), global_barrier, cred, global_barrier, gemv
+ *   CS_GEMVER_NAIVE     one fused group: segment<whole>(r2u, gtp), global_barrier, cred, global_barrier, gemv
  *   CS_GEMVER_TWOPASS   this build's form: Â never stored, A read twice (pass 1, cred, pass 2): three launches
- *   CS_GEMVER_TWOPASS_1 the same three stages fused with global barriers: one launch
+ *   CS_GEMVER_TWOPASS_1 the same three stages as one fused group with global barriers
  * Storage of A (store): 0 f32, 1 bf16, 2 e4m3 with a per-row f32 scale (TWOPASS styles only).
  * Inputs: A[i][j] = hash_unit(seed_A, i * n + j), u1 .. z likewise with their own seeds (base_seed * 16 + 1 .. 7, the
  * reference's seed_A .. seed_z for base_seed 0). alpha scales w, beta scales x.
@@ -59,7 +59,7 @@ int cs_gemver_destroy(cs_gemver g);
  * main.cpp:159-199) with this build's incompressible physics on a periodic nx x ny grid:
  *   Ru = u u_x + v u_y + p_x - nu lap u,  Rv = u v_x + v v_y + p_y - nu lap v,  Rc = u_x + v_y  (central differences),
  * then the FP64 sums of squares ("sq", 3 doubles) and L2 norms ("norm": |Ru|, |Rv|, |Rc|, all three; 4 doubles).
- *   CS_NS_FUSED    one launch: the tile stencil, a global barrier, the fixed-order reduction of the tile partials
+ *   CS_NS_FUSED    one fused group: the tile stencil, a global barrier, the fixed-order reduction of the tile partials
  *   CS_NS_UNFUSED  four launches: derivatives to global memory, combine, squares, norm
  * store: 0 f32 fields, 1 bf16 fields (arithmetic f32, norms FP64). tile_rows: rows per work-group of the fused stencil.
  * Fields: u, v, p from 8 random Fourier modes each (seeded), generated on the device. Names for cs_ns_read: "u" "v" "p"

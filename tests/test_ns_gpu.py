@@ -1,4 +1,4 @@
-"""The incompressible NS residual on the device, fused (one launch) and unfused, against the FP64 oracle."""
+"""The incompressible NS residual on the device, fused (one fused group) and unfused, against the FP64 oracle."""
 import ctypes as C
 
 import numpy as np
