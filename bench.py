@@ -63,12 +63,31 @@ class Gemver:
         self.check(self.w.cs_gemver_time_launches(self.h, iters, ms, 16, C.byref(cnt)))
         return [ms[k] for k in range(cnt.value)]
 
+    def stage_ms(self, leaf, iters):
+        ms = C.c_float()
+        self.check(self.w.cs_gemver_time_stage(self.h, leaf, iters, C.byref(ms)))
+        return ms.value
+
     def close(self):
         self.check(self.w.cs_gemver_destroy(self.h))
 
 
+def profiled_traffic(n, R, rt2):
+    """HBM bytes per launch of the two passes from the committed rocprofv3 counter summary of this exact configuration
+    (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "gemver_twopass_n16384_hbm_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes "):
+            kv = dict(t.split("=") for t in line.split()[1:])
+            if kv.get("config") == f"n{n}_R{R}_RT{rt2}":
+                return {"pass1": int(kv["pass1"]), "pass2": int(kv["pass2"])}
+    return None
+
+
 class Ns:
-    def __init__(self, style, n, store=0, ty=64):
+    def __init__(self, style, n, store=0, ty=32):
         from cospanmdao_amd import check, workloads
 
         self.w, self.check = workloads, check
@@ -228,8 +247,8 @@ def copy_calibration(gib=1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--R", type=int, default=128)
     ap.add_argument("--rt2", type=int, default=4)
@@ -254,38 +273,44 @@ def main():
 
     g = Gemver(style, n, R, rt2=a.rt2)
     ms = timed_steps(g, a.steps, a.warmup)
-    launch_ms = g.launches(max(5, a.steps // 2))
+    launch_ms = g.launches(5)
+    # the dominant kernel: pass 1 (reads A and u1 v1 u2 v2 y) or pass 2 (reads A and u1 v1 u2 v2 x, writes w), each timed
+    # back to back between two device events on the model's stream
+    stage_ms = [g.stage_ms(0, 20), g.stage_ms(2, 20)] if style == TWOPASS else None
     g.close()
     fused_bytes = gemver_bytes_fused(n)
     value = gbs(fused_bytes, ms)
-    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step, {value:.1f} GB/s; launches (us): {[round(t * 1e3, 1) for t in launch_ms]}")
+    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step, {value:.1f} GB/s; launches (us): {[round(t * 1e3, 1) for t in launch_ms]}"
+        f"; passes back to back (us): {[round(t * 1e3, 1) for t in stage_ms] if stage_ms else None}")
 
-    # the dominant kernel: pass 1 (A + u1 v1 u2 v2 y) or pass 2 (A + u1 v1 u2 v2 x + w)
-    if style == TWOPASS and len(launch_ms) == 3:
-        k = 0 if launch_ms[0] >= launch_ms[2] else 2
+    traffic = profiled_traffic(n, R, a.rt2)
+    if stage_ms:
+        k = 0 if stage_ms[0] >= stage_ms[1] else 1
         kbytes = n * n * 4 + (5 * n * 4 if k == 0 else 7 * n * 4)
         kname = "gemver pass 1 (AhatTPartial<f32>)" if k == 0 else f"gemver pass 2 (AhatGemv<f32,{a.rt2}>)"
-        kms = launch_ms[k]
+        kms = stage_ms[k]
+        ktraffic = traffic[("pass1", "pass2")[k]] if traffic else None
     else:
-        kbytes, kname, kms = fused_bytes, "gemver fused two-pass (one fused group)", launch_ms[0]
+        kbytes, kname, kms, ktraffic = fused_bytes, "gemver fused two-pass (one fused group)", ms, None
     achieved = gbs(kbytes, kms)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
-                "traffic": None, "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}
+                "traffic": ktraffic, "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}
 
-    extra = {"fused_ms": round(ms, 4), "fused_pct_of_peak": round(100 * value / PEAK_GBS, 2), "launch_ms": [round(t, 4) for t in launch_ms]}
+    extra = {"fused_ms": round(ms, 4), "fused_pct_of_peak": round(100 * value / PEAK_GBS, 2), "launch_ms": [round(t, 4) for t in launch_ms],
+             "pass_ms_back_to_back": [round(t, 4) for t in stage_ms] if stage_ms else None, "traffic_bytes": traffic}
     if not a.quick:
         p = Gemver(UNFUSED, n, 64)
-        pms = timed_steps(p, max(3, a.steps // 2), 2)
+        pms = timed_steps(p, max(3, a.steps // 4), 3)
         p.close()
         extra["plain"] = {"ms": round(pms, 4), "GBps_own_bytes": round(gbs(gemver_bytes_plain(n, 64), pms), 1),
                           "GBps_fused_bytes": round(gbs(fused_bytes, pms), 1), "fused_speedup_vs_plain": round(pms / ms, 3)}
         log(f"plain (4 components, B materialised): {pms * 1e3:.1f} us/step; fused speedup {pms / ms:.2f}x")
-        bench_ns(extra, a.steps)
+        bench_ns(extra, max(20, a.steps // 4))
         if a.batch:
             bench_batch(extra, value, max(30, a.steps * 3))
         else:
             extra["batched_ragged"] = "not measured: the persistent kernel's fix is not yet verified on a GPU (--batch runs it)"
-        bench_precision(extra, a.steps)
+        bench_precision(extra, max(20, a.steps // 8))
 
     cpu = None
     if not (a.no_cpu or a.quick):

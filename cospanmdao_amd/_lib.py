@@ -75,6 +75,7 @@ WORKLOADS_API = [
     ("cs_gemver_read", _I, [_P, C.c_char_p, _P, _S]),
     ("cs_gemver_time", _I, [_P, _I, C.POINTER(_F)]),
     ("cs_gemver_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
+    ("cs_gemver_time_stage", _I, [_P, _I, _I, C.POINTER(_F)]),
     ("cs_gemver_stats", _I, [_P, C.POINTER(C.c_uint64)]),
     ("cs_gemver_destroy", _I, [_P]),
     ("cs_ns_create", _I, [C.POINTER(_P), _I, _I, _S, _S, C.c_double, C.c_double, _U32, _I]),

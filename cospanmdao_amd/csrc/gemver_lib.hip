@@ -163,6 +163,13 @@ int cs_gemver_time_launches(cs_gemver g, int iters, float* ms, int cap, int* cou
     });
 }
 
+int cs_gemver_time_stage(cs_gemver g, int leaf, int iters, float* ms) {
+    return wl::guard([&] {
+        if (!g || !ms || leaf < 0 || iters <= 0) throw std::invalid_argument("cs_gemver_time_stage: bad argument");
+        *ms = model(*g).time_leaf(static_cast<std::size_t>(leaf), iters);
+    });
+}
+
 int cs_gemver_stats(cs_gemver g, uint64_t* stats4) {
     return wl::guard([&] {
         if (!g || !stats4) throw std::invalid_argument("cs_gemver_stats: null argument");

@@ -179,6 +179,44 @@ public:
         return out;
     }
 
+    /// The steady-state duration in milliseconds of leaf `k`'s launches: the leaves before it run once, then leaf k runs
+    /// `iters` times back to back on one stream between two device events (a forward stage is idempotent: it rewrites
+    /// the same outputs from the same inputs). Eager; no graph is captured or replayed by it.
+    float time_leaf(std::size_t k, int iters) {
+        detail::ModelMeta& m = *meta_;
+        if (iters <= 0) throw std::invalid_argument("naga::cospan: time_leaf: iters must be positive");
+        m.exec.init();
+        for (auto& h : m.hooks) h.apply(m, *h.inst->var);
+        std::vector<detail::LaunchRecord> recs;
+        for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
+        if (k >= recs.size())
+            throw std::invalid_argument("naga::cospan: time_leaf: the model has " + std::to_string(recs.size()) + " leaves, not " +
+                                        std::to_string(k + 1));
+        for (std::size_t j = 0; j < k; ++j) m.exec.submit(*m.leaves[j], recs[j]);
+        m.exec.synchronize();
+        const cs_hip_stream s = m.exec.stream(0);
+        cs_hip_event        a = nullptr, b = nullptr;
+        detail::hip_check(cs_hip_event_create(&a, 1), "event");
+        detail::hip_check(cs_hip_event_create(&b, 1), "event");
+        float ms = 0.0f;
+        try {
+            recs[k].submit(s);  // warm
+            detail::hip_check(cs_hip_event_record(a, s), "event record");
+            for (int i = 0; i < iters; ++i) recs[k].submit(s);
+            detail::hip_check(cs_hip_event_record(b, s), "event record");
+            detail::hip_check(cs_hip_event_synchronize(b), "leaf timing");
+            detail::hip_check(cs_hip_event_elapsed_ms(&ms, a, b), "leaf timing");
+        } catch (...) {
+            cs_hip_event_destroy(a);
+            cs_hip_event_destroy(b);
+            throw;
+        }
+        cs_hip_event_destroy(a);
+        cs_hip_event_destroy(b);
+        m.exec.synchronize();
+        return ms / static_cast<float>(iters);
+    }
+
     template <DType DT, int Rank>
     HostView<DT, Rank> view_output(std::string_view path) {
         return view<DT, Rank>(path, detail::IOKind::output);

@@ -17,8 +17,9 @@
 // the stencil, then the reduction, as two launches of one kernel); and an unfused variant of separate kernels with
 // intermediates in global memory (components.hpp:636-852).
 //
-// NsTile<S>: a work-group owns TY rows x (256 * V) columns; each work-item owns V consecutive columns (one 16-byte load
-// per field and row) and marches down its rows with a sliding window of three rows in registers, so every field element
+// NsTile<S>: a work-group owns TY rows x (256 * V) columns; each work-item owns V = 4 consecutive columns (one 16-byte
+// load per field and row for f32 fields, 8 bytes for bf16) and marches down its rows with a sliding window of three rows
+// in registers, so every field element
 // is read from HBM once (plus two halo rows per tile). The x neighbours of a work-item's V columns come from the adjacent
 // lanes by DPP wave shifts; only lanes 0 and 63 load their halo column from memory. Tiles are mapped to XCDs in
 // contiguous runs, so the halo rows of consecutive tiles meet in the same L2. S is the storage of u, v, p: f32 or bf16.
@@ -51,14 +52,16 @@ struct Field<DType::f32> {
     }
     __device__ static float one(const T& v) { return v; }
 };
+/// bf16 fields keep the f32 layout of work (4 columns per work-item: one 8-byte load per field and row), so the tile
+/// kernel has the same register footprint and occupancy as with f32 fields and simply reads half the bytes.
 template <>
 struct Field<DType::bf16> {
     using T                   = cs::bf16;
-    static constexpr int kVec = 8;
-    using Raw                 = std::uint32_t __attribute__((ext_vector_type(4)));
+    static constexpr int kVec = 4;
+    using Raw                 = std::uint32_t __attribute__((ext_vector_type(2)));
     __device__ static void widen(const Raw& r, float (&a)[kVec]) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 2; ++q) {
             a[2 * q]     = __uint_as_float(r[q] << 16);
             a[2 * q + 1] = __uint_as_float(r[q] & 0xffff0000u);
         }

@@ -50,6 +50,9 @@ int cs_gemver_time(cs_gemver g, int iters, float* ms_per_run);
 /* `iters` forwards with a device event around every launch: the average milliseconds of each launch of one forward, in
  * launch order (up to `cap` entries); *count = the launches of one forward */
 int cs_gemver_time_launches(cs_gemver g, int iters, float* ms, int cap, int* count);
+/* the steady-state milliseconds of leaf `leaf` (0-based, launch order) run `iters` times back to back between two device
+ * events (the leaves before it run once first) */
+int cs_gemver_time_stage(cs_gemver g, int leaf, int iters, float* ms);
 /* the model's executor counters: runs, kernel launches, graph captures, graph replays */
 int cs_gemver_stats(cs_gemver g, uint64_t* stats4);
 int cs_gemver_destroy(cs_gemver g);
