@@ -40,7 +40,7 @@ public:
 
     void sync() {
         detail::VarMeta& v = *var_;
-        if (v.write) detail::hip_check(cs_hip_event_synchronize(v.write->e), "view sync");
+        if (v.write) detail::hip_check(cs_hip_event_synchronize(v.write->get()), "view sync");
         for (int d = 0; d < Rank; ++d) shape_[d] = v.dims[d];
         data_.resize(v.elements());
         if (!data_.empty()) {
@@ -263,8 +263,10 @@ private:
     }
 
     /// After a graph launch on stream 0: every variable's last event is the graph's completion.
+    /// After a replay on stream 0: the graph's tail (a deferred event) is every written variable's writer and every read
+    /// variable's only reader (it subsumes earlier replays' reads, which ran before it on the same stream).
     void after_graph(detail::ModelMeta& m) {
-        const detail::EventPtr e = m.exec.record(0);
+        const detail::EventPtr e = m.exec.deferred(0);
         for (detail::Leaf* leaf : m.leaves) {
             for (detail::VarMeta* v : leaf->writes) {
                 v->write = e;
@@ -274,7 +276,10 @@ private:
         }
         for (detail::Leaf* leaf : m.leaves)
             for (detail::VarMeta* v : leaf->reads)
-                if (v->write != e) v->reads.push_back(e);
+                if (v->write != e) {
+                    v->reads.clear();
+                    v->reads.push_back(e);
+                }
     }
 
     std::unique_ptr<detail::ModelMeta> meta_;

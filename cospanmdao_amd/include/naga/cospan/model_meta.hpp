@@ -81,11 +81,20 @@ public:
         return ev;
     }
 
+    /// A deferred event standing for the work enqueued on stream s so far: no HIP event is created or recorded until a
+    /// wait on another stream or the host needs one (a graph replay then costs one graph launch and nothing else).
+    EventPtr deferred(int s) {
+        auto ev   = std::make_shared<Event>(nullptr, s);
+        ev->where = streams_[s];
+        last_[s]  = ev;
+        return ev;
+    }
+
     /// Stream s waits for `e` (nothing when e is null or already in s's order).
     void wait(int s, const EventPtr& e) {
         if (!e || e->stream == s) return;
         join_capture(s);
-        hip_check(cs_hip_stream_wait_event(streams_[s], e->e), "stream wait");
+        hip_check(cs_hip_stream_wait_event(streams_[s], e->get()), "stream wait");
     }
 
     /// The dependencies of a launch that reads `reads` and writes `writes`.
@@ -141,7 +150,11 @@ public:
         use_[s] = ++clock_;
         ++launches_;
         const EventPtr ev = record(s);
-        for (VarMeta* v : leaf.reads) v->reads.push_back(ev);
+        for (VarMeta* v : leaf.reads) {
+            // a read on stream s subsumes the earlier reads on s (one stream runs in order): at most one per stream
+            std::erase_if(v->reads, [s](const EventPtr& r) { return r->stream == s; });
+            v->reads.push_back(ev);
+        }
         for (VarMeta* v : leaf.writes) {
             v->write = ev;
             v->reads.clear();

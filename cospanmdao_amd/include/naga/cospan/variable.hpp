@@ -151,13 +151,23 @@ namespace detail {
 
 /// A recorded HIP event and the executor stream it was recorded on; destroyed with its last holder.
 struct Event {
-    cs_hip_event e      = nullptr;
-    int          stream = 0;
+    cs_hip_event  e      = nullptr;
+    int           stream = 0;
+    cs_hip_stream where  = nullptr;  ///< a deferred event: recorded on this stream when first needed
     Event(cs_hip_event ev, int s) : e{ev}, stream{s} {}
     Event(const Event&)            = delete;
     Event& operator=(const Event&) = delete;
     ~Event() {
         if (e) cs_hip_event_destroy(e);
+    }
+    /// The HIP event. A deferred event (the tail of a graph replay) is recorded on its stream now: it then covers at least
+    /// the work it stands for, since that stream runs in order.
+    cs_hip_event get() {
+        if (!e) {
+            hip_check(cs_hip_event_create(&e, 0), "event");
+            hip_check(cs_hip_event_record(e, where), "event record");
+        }
+        return e;
     }
 };
 using EventPtr = std::shared_ptr<Event>;
@@ -221,8 +231,8 @@ struct VarMeta {
     }
     /// Waits (on the host) for every event, then frees the allocation.
     void release() {
-        if (write) cs_hip_event_synchronize(write->e);
-        for (auto& r : reads) cs_hip_event_synchronize(r->e);
+        if (write) cs_hip_event_synchronize(write->get());
+        for (auto& r : reads) cs_hip_event_synchronize(r->get());
         if (data) hip_check(cs_hip_free(data), "free of '" + name + "'");
         data     = nullptr;
         capacity = 0;
