@@ -166,7 +166,7 @@ public:
             return reinterpret_cast<const void*>(&index_kernel<X...>);
     }
 
-    /// Launches the group on `s`. `split`: the range_split strategy; `waits`: group_waits_enabled or a global_barrier.
+    /// Launches the group on `s`. `split`: the range_split strategy; `waits`: group_waits_enabled.
     void launch(Split split, bool waits, const std::vector<SegmentPlan>& segs, const StagePlan (&stages)[N], const ArgPack<X...>& args,
                 cs_hip_stream s, const std::string& group_name) {
         prepare(split, waits, segs, stages, args, s, group_name).submit(s);
@@ -201,19 +201,21 @@ public:
     }
 
 private:
-    /// One launch running segments [a, b) of the group (their indices renumbered from 0; the other stages never run).
-    LaunchRecord prepare_run(Split split, bool waits, const std::vector<SegmentPlan>& all, std::size_t a, std::size_t b,
+    /// One launch running segments [a, b) of the group. The segment table always holds every segment with the indices,
+    /// group counts and first tickets the stages see (get_segment, get_segment_count, get_segment_group_count); the launch
+    /// covers the groups of [a, b) only.
+    LaunchRecord prepare_run(Split split, bool waits, const std::vector<SegmentPlan>& segs, std::size_t a, std::size_t b,
                              const StagePlan (&stages)[N], const ArgPack<X...>& args, cs_hip_stream s, const std::string& group_name) {
         LaunchRecord    rec;
         rec.name = group_name;
         GroupArgs<X...> ga{};
         ga.args                 = args;
         Geometry<N>&  geo       = ga.geo;
-        const std::vector<SegmentPlan> segs(all.begin() + static_cast<std::ptrdiff_t>(a), all.begin() + static_cast<std::ptrdiff_t>(b));
         const auto    nseg      = static_cast<std::uint32_t>(segs.size());
         geo.segments = nseg;
+        geo.seg_lo   = static_cast<std::uint32_t>(a);
+        geo.seg_hi   = static_cast<std::uint32_t>(b);
         geo.split    = static_cast<std::uint32_t>(split);
-        for (int k = 0; k < N; ++k) geo.stage[k].segment = kNoSegment;
 
         // the one work-group shape of the group (group-range stages)
         std::uint32_t wg[3] = {1, 1, 1};
@@ -234,8 +236,8 @@ private:
                                             std::to_string(kMaxThreads));
         }
 
-        // segment table, LDS layout
-        std::uint64_t first_ticket = 0, offset = 0, lds = 0;
+        // segment table (all segments), LDS layout; the launch's extent from segments [a, b)
+        std::uint64_t first_ticket = 0, offset = 0, lds = 0, run_groups = 0, run_offset = 0;
         std::uint64_t maxr[3]      = {0, 0, 0};
         for (std::uint32_t si = 0; si < nseg; ++si) {
             const SegmentPlan& sp = segs[si];
@@ -263,42 +265,47 @@ private:
             info.boundary = static_cast<std::uint64_t>(sp.boundary);
             first_ticket += info.count;
             offset += info.range[2];
-            for (int d = 0; d < 3; ++d)
-                if (info.range[d] > maxr[d]) maxr[d] = info.range[d];
-            if (seg_lds > lds) lds = seg_lds;
+            if (si >= a && si < b) {
+                run_groups += info.count;
+                run_offset += info.range[2];
+                for (int d = 0; d < 3; ++d)
+                    if (info.range[d] > maxr[d]) maxr[d] = info.range[d];
+                if (seg_lds > lds) lds = seg_lds;
+            }
         }
-        geo.total_groups = first_ticket;
+        // the bound of a linear group index in this launch (tickets and flattened ids count from segment 0)
+        geo.total_groups = geo.seg[b - 1].first + geo.seg[b - 1].count;
         if (lds > 160u * 1024u)
             throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': " + std::to_string(lds) +
                                         " bytes of local memory exceed the 163840 of a CDNA4 work-group");
-        if (geo.total_groups == 0) return rec;  // nothing to run: an empty record
+        if (run_groups == 0) return rec;  // nothing to run: an empty record
 
         // physical grid (x is physical dimension 2); a hardware grid allows 65535 groups in y and z, beyond that the
         // groups are flattened along x (the logical coordinates are the same either way)
-        if (split != Split::flatten && (maxr[0] > 65535 || maxr[1] > 65535 || nseg > 65535)) {
+        if (split != Split::flatten && (maxr[0] > 65535 || maxr[1] > 65535 || b - a > 65535)) {
             split     = Split::flatten;
             geo.split = static_cast<std::uint32_t>(split);
         }
         std::uint32_t grid[3] = {1, 1, 1};
         if constexpr (kIndexRange) {
-            grid[0] = static_cast<std::uint32_t>((geo.total_groups + 255) / 256);
+            grid[0] = static_cast<std::uint32_t>((run_groups + 255) / 256);
             wg[2]   = 256;
             if (waits) throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': index-range stages cannot wait");
             geo.split = static_cast<std::uint32_t>(Split::flatten);
         } else if (waits || split == Split::flatten) {
-            if (geo.total_groups > 0x7fffffffull)
+            if (run_groups > 0x7fffffffull)
                 throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': too many work-groups for one launch");
-            grid[0] = static_cast<std::uint32_t>(geo.total_groups);
+            grid[0] = static_cast<std::uint32_t>(run_groups);
             if (waits) geo.split = static_cast<std::uint32_t>(Split::flatten);
         } else if (split == Split::pad) {
-            grid[0] = static_cast<std::uint32_t>(offset);
+            grid[0] = static_cast<std::uint32_t>(run_offset);
             grid[1] = static_cast<std::uint32_t>(maxr[1]);
             grid[2] = static_cast<std::uint32_t>(maxr[0]);
         } else {
             if (maxr[0] > 1) throw std::invalid_argument("naga::cospan: fused group '" + group_name + "': range_split::padded needs stages of rank 2 or lower");
             grid[0] = static_cast<std::uint32_t>(maxr[2]);
             grid[1] = static_cast<std::uint32_t>(maxr[1]);
-            grid[2] = nseg;
+            grid[2] = static_cast<std::uint32_t>(b - a);
         }
 
         if (waits) geo.ticket_state = tickets_.ensure(ticket_words(nseg, geo.total_groups), s);

@@ -85,6 +85,8 @@ struct Geometry {
     std::uint64_t  total_groups = 0;
     std::uint32_t  segments     = 0;
     std::uint32_t  split        = 0;
+    std::uint32_t  seg_lo       = 0;  ///< the segments this launch runs: [seg_lo, seg_hi) (all of them but for a group
+    std::uint32_t  seg_hi       = 0;  ///< whose global barriers are launch boundaries)
     StageGeometry  stage[N];
     SegmentInfo    seg[N];
 };
@@ -157,27 +159,30 @@ __device__ inline LogicalGroup locate_linear(const SegmentInfo* tbl, std::uint32
 
 /// Hardware-direct mapping (no tickets), per split strategy (reference kernel.hpp:266-288). Physical dimension 2 is
 /// HIP's x, 1 is y, 0 is z.
-__device__ inline LogicalGroup locate_physical(std::uint32_t split, const SegmentInfo* tbl, std::uint32_t segments,
+/// Segments [lo, hi) are the ones this launch runs; the physical ids count from segment lo.
+__device__ inline LogicalGroup locate_physical(std::uint32_t split, const SegmentInfo* tbl, std::uint32_t lo, std::uint32_t hi,
                                                std::uint64_t total) {
     const std::uint64_t d0 = blockIdx.z, d1 = blockIdx.y, d2 = blockIdx.x;
-    if (split == static_cast<std::uint32_t>(Split::flatten)) return locate_linear(tbl, segments, d2, total);
+    if (split == static_cast<std::uint32_t>(Split::flatten)) return locate_linear(tbl, hi, tbl[lo].first + d2, total);
     LogicalGroup r;
     if (split == static_cast<std::uint32_t>(Split::pad)) {
-        for (std::uint32_t s = 0; s < segments; ++s) {
-            if (d2 >= tbl[s].offset && d2 < tbl[s].offset + tbl[s].range[2]) {
+        const std::uint64_t x = tbl[lo].offset + d2;
+        for (std::uint32_t s = lo; s < hi; ++s) {
+            if (x >= tbl[s].offset && x < tbl[s].offset + tbl[s].range[2]) {
                 if (d0 < tbl[s].range[0] && d1 < tbl[s].range[1]) {
                     r.segment  = s;
                     r.coord[0] = d0;
                     r.coord[1] = d1;
-                    r.coord[2] = d2 - tbl[s].offset;
+                    r.coord[2] = x - tbl[s].offset;
                     r.valid    = true;
                 }
                 break;
             }
         }
     } else {
-        if (d0 < segments && d1 < tbl[d0].range[1] && d2 < tbl[d0].range[2]) {
-            r.segment  = static_cast<std::uint32_t>(d0);
+        const std::uint64_t s = lo + d0;
+        if (s < hi && d1 < tbl[s].range[1] && d2 < tbl[s].range[2]) {
+            r.segment  = static_cast<std::uint32_t>(s);
             r.coord[0] = 0;
             r.coord[1] = d1;
             r.coord[2] = d2;
@@ -190,8 +195,6 @@ __device__ inline LogicalGroup locate_physical(std::uint32_t split, const Segmen
 // ---- ticket state ---------------------------------------------------------------------------------------------------
 // [0] ticket counter, [1] finished counter, [2, 2+S) per-segment finished counts, [2+S, 2+S+T) per-ticket done flags.
 inline constexpr std::size_t kTicketHeader = 2;
-/// The segment of a stage that does not run in a launch (a fused group split at its global barriers).
-inline constexpr std::uint32_t kNoSegment = 0xffffffffu;
 CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return kTicketHeader + segments + total; }
 
 /// A spin-wait load: relaxed at agent scope (coherent across the XCDs' L2s without invalidating them on every poll); the
@@ -312,14 +315,19 @@ struct GroupItem {
         return detail::tail_range<R>(table_[s].range[0], table_[s].range[1], table_[s].range[2]);
     }
 
-    /// COLLECTIVE: wait until group `g` of the EARLIER segment `s` has finished all its stages (ticket mode only).
+    /// COLLECTIVE: wait until group `g` of the EARLIER segment `s` has finished all its stages. Without wait state (a
+    /// group whose global barriers are launch boundaries) an earlier segment behind a barrier ran in an earlier launch and
+    /// has finished; waits between groups of one launch need cs::group_waits_enabled.
     __device__ void wait_group(std::size_t s, const id<R>& g) const {
+        if (ticket_state_ == nullptr) return;
         const id<3>         l = detail::lift_id<R>(g);
         const std::uint64_t c[3] = {l[0], l[1], l[2]};
         detail::ticket_wait_group(ticket_state_, segments_, table_, static_cast<std::uint32_t>(s), c);
     }
     /// COLLECTIVE: wait until every group of every earlier segment has finished (what cs::global_barrier() does).
-    __device__ void wait_earlier_segments() const { detail::ticket_wait_segments(ticket_state_, table_, segment_); }
+    __device__ void wait_earlier_segments() const {
+        if (ticket_state_ != nullptr) detail::ticket_wait_segments(ticket_state_, table_, segment_);
+    }
 
     __device__ Group<R> get_group() const { return Group<R>{group_id_, group_range_}; }
     __device__ void     barrier() const { __syncthreads(); }
@@ -683,7 +691,7 @@ __global__ __launch_bounds__(MaxThreads) void group_kernel(const GroupArgs<X...>
         if (geo.seg[lg.segment].boundary == static_cast<std::uint64_t>(Boundary::global_barrier))
             ticket_wait_segments(ts, geo.seg, lg.segment);
     } else {
-        lg = locate_physical(geo.split, geo.seg, geo.segments, geo.total_groups);
+        lg = locate_physical(geo.split, geo.seg, geo.seg_lo, geo.seg_hi, geo.total_groups);
         if (!lg.valid) return;
     }
 
@@ -729,8 +737,8 @@ __device__ inline void run_index_stages(const GroupArgs<X...>& ga, const Logical
 template <class... X>
 __global__ __launch_bounds__(256) void index_kernel(const GroupArgs<X...> ga) {
     constexpr int       N   = sizeof...(X);
-    const std::uint64_t lin = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const LogicalGroup  lg  = locate_linear(ga.geo.seg, ga.geo.segments, lin, ga.geo.total_groups);
+    const std::uint64_t lin = ga.geo.seg[ga.geo.seg_lo].first + static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const LogicalGroup  lg  = locate_linear(ga.geo.seg, ga.geo.seg_hi, lin, ga.geo.total_groups);
     if (!lg.valid) return;
     run_index_stages<N>(ga, lg, std::make_index_sequence<N>{});
 }

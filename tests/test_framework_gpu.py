@@ -1,0 +1,20 @@
+"""The fused-group machinery through cs::Model on the GPU: tests/framework/fused_groups.cpp (segments, range modes, local
+memory, work-group shape errors, bare components, group waits, global barriers, cross-group reads over several forwards),
+built by __graft_entry__.build() / make -C tests/framework."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_fused_groups_program():
+    exe = os.path.join(HERE, "framework", "_build", "fused_groups")
+    assert os.path.exists(exe), "tests/framework/_build/fused_groups is not built (make -C tests/framework)"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
