@@ -77,6 +77,20 @@ __device__ inline T readlane(T x, int l) {
     }
 }
 
+/// x of lane `src` (any trivially copyable value, moved as 32-bit words through ds_bpermute). `src` must be an active lane.
+template <class T>
+__device__ inline T shfl(T x, int src) {
+    static_assert(std::is_trivially_copyable_v<T>, "cospan: shuffles move trivially copyable values");
+    constexpr int kWords = static_cast<int>((sizeof(T) + 3) / 4);
+    std::uint32_t w[kWords] = {};
+    __builtin_memcpy(w, &x, sizeof(T));
+    for (int k = 0; k < kWords; ++k)
+        w[k] = static_cast<std::uint32_t>(__builtin_amdgcn_ds_bpermute(src << 2, static_cast<int>(w[k])));
+    T r;
+    __builtin_memcpy(&r, w, sizeof(T));
+    return r;
+}
+
 /// x of lane (lane ^ mask), through ds_bpermute.
 template <class T>
 __device__ inline T shfl_xor(T x, int mask) {
@@ -99,6 +113,21 @@ __device__ inline T reduce(T x, Op op) {
     return x;
 }
 
+/// Reduction over lanes 0..n-1 (n uniform, 1 <= n <= 64; lanes n..63 may be inactive or hold anything): the same balanced
+/// pairwise tree in lane order as reduce(), built by strided steps that only read lanes below n, then broadcast from
+/// lane 0. Every active lane gets the result.
+template <class T, class Op>
+__device__ inline T reduce_first(T x, Op op, int n) {
+    if (n == kSize) return reduce(x, op);
+    const int l = lane_id();
+    for (int width = 1; width < n; width <<= 1) {
+        const int src = l + width < n ? l + width : l;
+        const T   y   = shfl(x, src);
+        if ((l & (2 * width - 1)) == 0 && l + width < n) x = op(x, y);
+    }
+    return shfl(x, 0);
+}
+
 }  // namespace naga::cospan::wave
 
 namespace naga::cospan::block {
@@ -107,19 +136,32 @@ namespace naga::cospan::block {
 /// pairwise tree in wave order. Every work-item gets the result. Uses one LDS slot per wave; ends with a barrier so the
 /// slots may be reused by the next call.
 template <class T, class Op>
+__device__ inline T reduce_first(T x, Op op, int count);
+
+template <class T, class Op>
 __device__ inline T reduce(T x, Op op) {
+    return reduce_first(x, op, static_cast<int>(blockDim.x * blockDim.y * blockDim.z));
+}
+
+/// The same over the values of work-items 0..count-1 in local linear order (count uniform, 1 <= count <= the group
+/// size); the other work-items take part and get the result too. A wave that is only partly populated (a group size that
+/// is not a multiple of 64) never reads its missing lanes.
+template <class T, class Op>
+__device__ inline T reduce_first(T x, Op op, int count) {
     __shared__ T slots[16];
-    const int    nthreads = static_cast<int>(blockDim.x * blockDim.y * blockDim.z);
-    const int    tid      = static_cast<int>(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
-    const int    nwaves   = (nthreads + wave::kSize - 1) / wave::kSize;
-    x                     = wave::reduce(x, op);
-    if (nwaves == 1) return x;
-    if ((tid & (wave::kSize - 1)) == 0) slots[tid / wave::kSize] = x;
+    const int    tid    = static_cast<int>(threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z));
+    const int    w      = tid / wave::kSize;
+    const int    nwaves = (count + wave::kSize - 1) / wave::kSize;
+    const int    in_w   = count - w * wave::kSize;   // values held by this wave (may be <= 0 past the count)
+    if (in_w > 0) x = wave::reduce_first(x, op, in_w < wave::kSize ? in_w : wave::kSize);
+    const int nthreads = static_cast<int>(blockDim.x * blockDim.y * blockDim.z);
+    if (nthreads <= wave::kSize) return x;   // one wave: reduce_first already broadcast lane 0's result
+    if ((tid & (wave::kSize - 1)) == 0 && w < nwaves) slots[w] = x;
     __syncthreads();
     T v[16];
-    for (int w = 0; w < nwaves; ++w) v[w] = slots[w];
+    for (int k = 0; k < nwaves; ++k) v[k] = slots[k];
     for (int width = 1; width < nwaves; width <<= 1)
-        for (int w = 0; w + width < nwaves; w += 2 * width) v[w] = op(v[w], v[w + width]);
+        for (int k = 0; k + width < nwaves; k += 2 * width) v[k] = op(v[k], v[k + width]);
     __syncthreads();
     return v[0];
 }

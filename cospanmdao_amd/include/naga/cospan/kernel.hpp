@@ -268,6 +268,32 @@ struct Group {
 };
 
 // -----------------------------------------------------------------------------
+// sub_group: one wavefront of 64 lanes (reference kernel.hpp:306 aliases sycl::sub_group)
+// -----------------------------------------------------------------------------
+/// The work-items of one wave, in local linear order: wave k of a work-group holds local linear ids 64k .. 64k+63 (the
+/// last wave of a group whose size is not a multiple of 64 holds fewer).
+struct sub_group {
+    using id_type        = id<1>;
+    using range_type     = range<1>;
+    using linear_id_type = std::uint32_t;
+
+    __device__ id<1>          get_local_id() const { return id<1>{static_cast<std::size_t>(get_local_linear_id())}; }
+    __device__ linear_id_type get_local_linear_id() const { return detail::local_linear_id() % wave::kSize; }
+    __device__ range<1>       get_local_range() const { return range<1>{static_cast<std::size_t>(get_local_linear_range())}; }
+    __device__ linear_id_type get_local_linear_range() const {
+        const std::uint32_t rest = detail::local_linear_range() - get_group_linear_id() * wave::kSize;
+        return rest < wave::kSize ? rest : wave::kSize;
+    }
+    __device__ range<1>       get_max_local_range() const { return range<1>{static_cast<std::size_t>(wave::kSize)}; }
+    __device__ id<1>          get_group_id() const { return id<1>{static_cast<std::size_t>(get_group_linear_id())}; }
+    __device__ linear_id_type get_group_linear_id() const { return detail::local_linear_id() / wave::kSize; }
+    __device__ range<1>       get_group_range() const {
+        return range<1>{static_cast<std::size_t>((detail::local_linear_range() + wave::kSize - 1) / wave::kSize)};
+    }
+    __device__ bool leader() const { return get_local_linear_id() == 0; }
+};
+
+// -----------------------------------------------------------------------------
 // GroupItem<R>: what a group-range body receives (reference kernel.hpp:364-446)
 // -----------------------------------------------------------------------------
 template <int R>
@@ -329,8 +355,9 @@ struct GroupItem {
         if (ticket_state_ != nullptr) detail::ticket_wait_segments(ticket_state_, table_, segment_);
     }
 
-    __device__ Group<R> get_group() const { return Group<R>{group_id_, group_range_}; }
-    __device__ void     barrier() const { __syncthreads(); }
+    __device__ Group<R>  get_group() const { return Group<R>{group_id_, group_range_}; }
+    __device__ sub_group get_sub_group() const { return sub_group{}; }
+    __device__ void      barrier() const { __syncthreads(); }
 
     id<R>                      group_id_;
     range<R>                   group_range_;
@@ -455,6 +482,14 @@ __device__ inline T group_broadcast(const Group<R>&, T x, std::size_t local_line
     __syncthreads();
     return v;
 }
+/// From the work-item at local id `local_id` (row-major over the local range, last dimension fastest).
+template <int R, class T>
+__device__ inline T group_broadcast(const Group<R>& g, T x, const id<R>& local_id) {
+    const range<R> lr = g.get_local_range();
+    std::size_t    l  = 0;
+    for (int d = 0; d < R; ++d) l = l * lr[d] + local_id[d];
+    return group_broadcast(g, x, l);
+}
 
 template <int R>
 __device__ inline bool any_of_group(const Group<R>&, bool pred) {
@@ -494,6 +529,110 @@ __device__ inline T inclusive_scan_over_group(const Group<R>&, T x, Op op) {
 template <int R, class V, class Op, class T>
 __device__ inline T inclusive_scan_over_group(const Group<R>&, V x, Op op, T init) {
     return op(init, op(block::exclusive_scan(static_cast<T>(x), op, detail::identity_of<Op, T>()), static_cast<T>(x)));
+}
+
+// ---- joint algorithms over a range [first, last) that the whole group reads (reference kernel.hpp:614-676) ----
+// Work-item t takes elements t, t + size, t + 2 size, ... (size = the group's work-item count); the range and the
+// results are the same for every work-item. All are COLLECTIVE.
+
+/// Reduction of *first .. *(last-1); an empty range gives the operator's identity.
+template <int R, class Ptr, class Op>
+__device__ inline auto joint_reduce(const Group<R>&, Ptr first, Ptr last, Op op) {
+    using T              = std::remove_cv_t<std::remove_reference_t<decltype(*first)>>;
+    const std::ptrdiff_t n = last - first;
+    if (n <= 0) return detail::identity_of<Op, T>();
+    const std::ptrdiff_t size = static_cast<std::ptrdiff_t>(detail::local_linear_range());
+    const std::ptrdiff_t t    = static_cast<std::ptrdiff_t>(detail::local_linear_id());
+    T                    acc  = t < n ? static_cast<T>(first[t]) : detail::identity_of<Op, T>();
+    for (std::ptrdiff_t i = t + size; i < n; i += size) acc = op(acc, static_cast<T>(first[i]));
+    return block::reduce_first(acc, op, static_cast<int>(n < size ? n : size));
+}
+template <int R, class Ptr, class T, class Op>
+__device__ inline T joint_reduce(const Group<R>& g, Ptr first, Ptr last, T init, Op op) {
+    if (last - first <= 0) return init;
+    return op(init, static_cast<T>(joint_reduce(g, first, last, op)));
+}
+
+template <int R, class Ptr, class Pred>
+__device__ inline bool joint_any_of(const Group<R>& g, Ptr first, Ptr last, Pred pred) {
+    const std::ptrdiff_t n = last - first, size = static_cast<std::ptrdiff_t>(detail::local_linear_range());
+    bool                 hit = false;
+    for (std::ptrdiff_t i = static_cast<std::ptrdiff_t>(detail::local_linear_id()); i < n && !hit; i += size) hit = pred(first[i]);
+    return any_of_group(g, hit);
+}
+template <int R, class Ptr, class Pred>
+__device__ inline bool joint_all_of(const Group<R>& g, Ptr first, Ptr last, Pred pred) {
+    const std::ptrdiff_t n = last - first, size = static_cast<std::ptrdiff_t>(detail::local_linear_range());
+    bool                 ok = true;
+    for (std::ptrdiff_t i = static_cast<std::ptrdiff_t>(detail::local_linear_id()); i < n && ok; i += size) ok = pred(first[i]);
+    return all_of_group(g, ok);
+}
+template <int R, class Ptr, class Pred>
+__device__ inline bool joint_none_of(const Group<R>& g, Ptr first, Ptr last, Pred pred) {
+    return !joint_any_of(g, first, last, pred);
+}
+
+namespace detail {
+/// The joint scans: the range in chunks of one element per work-item, each chunk scanned across the group and offset by
+/// the running total of the chunks before it. `carry` starts as the init (or the identity). Returns result + n.
+template <bool Inclusive, class InPtr, class OutPtr, class T, class Op>
+__device__ inline OutPtr joint_scan(InPtr first, InPtr last, OutPtr result, T carry, Op op) {
+    const std::ptrdiff_t n = last - first, size = static_cast<std::ptrdiff_t>(local_linear_range());
+    const std::ptrdiff_t t  = static_cast<std::ptrdiff_t>(local_linear_id());
+    const T              id = identity_of<Op, T>();
+    for (std::ptrdiff_t base = 0; base < n; base += size) {
+        const bool in    = base + t < n;
+        const T    x     = in ? static_cast<T>(first[base + t]) : id;
+        T          total = id;
+        const T    ex    = block::exclusive_scan(x, op, id, &total);   // ends with a barrier: every read above is done
+        if (in) result[base + t] = Inclusive ? op(carry, op(ex, x)) : op(carry, ex);
+        carry = op(carry, total);
+    }
+    return result + (n > 0 ? n : 0);
+}
+}  // namespace detail
+
+template <int R, class InPtr, class OutPtr, class Op>
+__device__ inline OutPtr joint_exclusive_scan(const Group<R>&, InPtr first, InPtr last, OutPtr result, Op op) {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(*first)>>;
+    return detail::joint_scan<false>(first, last, result, detail::identity_of<Op, T>(), op);
+}
+template <int R, class InPtr, class OutPtr, class T, class Op>
+__device__ inline OutPtr joint_exclusive_scan(const Group<R>&, InPtr first, InPtr last, OutPtr result, T init, Op op) {
+    return detail::joint_scan<false>(first, last, result, init, op);
+}
+template <int R, class InPtr, class OutPtr, class Op>
+__device__ inline OutPtr joint_inclusive_scan(const Group<R>&, InPtr first, InPtr last, OutPtr result, Op op) {
+    using T = std::remove_cv_t<std::remove_reference_t<decltype(*first)>>;
+    return detail::joint_scan<true>(first, last, result, detail::identity_of<Op, T>(), op);
+}
+template <int R, class InPtr, class OutPtr, class Op, class T>
+__device__ inline OutPtr joint_inclusive_scan(const Group<R>&, InPtr first, InPtr last, OutPtr result, Op op, T init) {
+    return detail::joint_scan<true>(first, last, result, init, op);
+}
+
+// ---- the sub-group (wave) shuffles (reference kernel.hpp:679-693) ----
+// A source lane outside the sub-group's populated lanes gives the lane's own value (the result is unspecified there).
+
+template <class T>
+__device__ inline T select_from_group(sub_group g, T x, sub_group::id_type remote_local_id) {
+    const std::uint32_t src = static_cast<std::uint32_t>(remote_local_id[0]);
+    return wave::shfl(x, static_cast<int>(src < g.get_local_linear_range() ? src : g.get_local_linear_id()));
+}
+template <class T>
+__device__ inline T shift_group_left(sub_group g, T x, sub_group::linear_id_type delta = 1) {
+    const std::uint32_t src = g.get_local_linear_id() + delta;
+    return wave::shfl(x, static_cast<int>(src < g.get_local_linear_range() ? src : g.get_local_linear_id()));
+}
+template <class T>
+__device__ inline T shift_group_right(sub_group g, T x, sub_group::linear_id_type delta = 1) {
+    const std::uint32_t l = g.get_local_linear_id();
+    return wave::shfl(x, static_cast<int>(l >= delta ? l - delta : l));
+}
+template <class T>
+__device__ inline T permute_group_by_xor(sub_group g, T x, sub_group::linear_id_type mask) {
+    const std::uint32_t src = g.get_local_linear_id() ^ mask;
+    return wave::shfl(x, static_cast<int>(src < g.get_local_linear_range() ? src : g.get_local_linear_id()));
 }
 
 // -----------------------------------------------------------------------------

@@ -147,7 +147,134 @@ private:
     bool        local_;
 };
 
+/// Every group-function family over one work-group of W0 x W1 work-items (x = local linear id + 1) and a joint range of M
+/// local-memory floats (L[i] = i + 1), after the reference's tests/test_range_split.cpp:284-343. r[0..31]: group-wide
+/// results (leader); r[32 + 6 t + k]: work-item t's scans; r[32 + 6 n + i], r[32 + 6 n + M + i]: the joint exclusive and
+/// inclusive scans of the range.
+template <std::size_t W0, std::size_t W1, std::size_t M>
+class FamStage : public cs::Component<FamStage<W0, W1, M>> {
+public:
+    static constexpr std::size_t n = W0 * W1;
+    struct Inputs {};
+    struct Outputs {
+        cs::Output<DType::f32, 1> r;
+    };
+    static void compute_forward(cs::GroupItem<2> it, const Inputs&, const Outputs& out, TileLocal lm) {
+        auto              g   = it.get_group();
+        auto [Lv]             = lm.get(it);
+        float*            L   = &Lv[std::size_t{0}];
+        const std::size_t t   = it.get_local_linear_id();
+        const float       x   = static_cast<float>(t + 1);
+        const auto        pls = cs::plus<float>();
+        for (std::size_t i = t; i < M; i += n) L[i] = static_cast<float>(i + 1);
+        cs::group_barrier(g);
+
+        const float red      = cs::reduce_over_group(g, x, pls);
+        const float red_init = cs::reduce_over_group(g, x, 10.0f, pls);
+        const float red_max  = cs::reduce_over_group(g, x, cs::maximum<float>());
+        const float red_min  = cs::reduce_over_group(g, x, cs::minimum<float>());
+        const float bc0      = cs::group_broadcast(g, x);
+        const float bc_lin   = cs::group_broadcast(g, x, std::size_t{5});
+        const float bc_id    = cs::group_broadcast(g, x, cs::id<2>{1, 2});
+        const bool  any_t    = cs::any_of_group(g, x == static_cast<float>(n));
+        const bool  any_f    = cs::any_of_group(g, x > 100000.0f);
+        const bool  all_t    = cs::all_of_group(g, x > 0.0f);
+        const bool  none_t   = cs::none_of_group(g, x > static_cast<float>(n));
+        const float ex       = cs::exclusive_scan_over_group(g, x, pls);
+        const float in       = cs::inclusive_scan_over_group(g, x, pls);
+        const float ex_i     = cs::exclusive_scan_over_group(g, x, 100.0f, pls);
+        const float in_i     = cs::inclusive_scan_over_group(g, x, pls, 100.0f);
+        const float jred     = cs::joint_reduce(g, L, L + M, pls);
+        const float jred_i   = cs::joint_reduce(g, L, L + M, 1.0f, pls);
+        const float jmin     = cs::joint_reduce(g, L, L + M, cs::minimum<float>());
+        const bool  jany     = cs::joint_any_of(g, L, L + M, [](float v) { return v == static_cast<float>(M); });
+        const bool  jall     = cs::joint_all_of(g, L, L + M, [](float v) { return v > 0.0f; });
+        const bool  jnone    = cs::joint_none_of(g, L, L + M, [](float v) { return v > static_cast<float>(M); });
+        float* const jout    = &out.r[std::size_t{32 + 6 * n}];
+        cs::joint_exclusive_scan(g, L, L + M, L + M, pls);
+        cs::group_barrier(g);
+        const float jex = t < M ? L[M + t] : 0.0f;
+        for (std::size_t i = t; i < M; i += n) jout[i] = L[M + i];
+        cs::group_barrier(g);
+        cs::joint_inclusive_scan(g, L, L + M, L + M, pls);
+        cs::group_barrier(g);
+        const float jin = t < M ? L[M + t] : 0.0f;
+        for (std::size_t i = t; i < M; i += n) jout[M + i] = L[M + i];
+
+        cs::sub_group sg  = it.get_sub_group();
+        const float   px  = cs::permute_group_by_xor(sg, x, 0u);
+        const float   sel = cs::select_from_group(sg, x, sg.get_local_id());
+        const float   sl  = cs::shift_group_left(sg, x, 0);
+        const float   sr  = cs::shift_group_right(sg, x, 0);
+        const float   px1 = cs::permute_group_by_xor(sg, x, 1u);
+        const float   sel3 = cs::select_from_group(sg, x, cs::id<1>{3});
+        const float   sl1 = cs::shift_group_left(sg, x);
+        const float   sr1 = cs::shift_group_right(sg, x);
+
+        out.r[32 + t * 6 + 0] = ex;
+        out.r[32 + t * 6 + 1] = in;
+        out.r[32 + t * 6 + 2] = ex_i;
+        out.r[32 + t * 6 + 3] = in_i;
+        out.r[32 + t * 6 + 4] = jex;
+        out.r[32 + t * 6 + 5] = jin;
+        // the wave shuffles by 1 where the source lane is inside the item's own wave: r[24..27] from work-item 5
+        if (t == 5) {
+            out.r[24] = px1;   // lane 5 ^ 1 = 4 -> x 5
+            out.r[25] = sel3;  // lane 3 -> x 4
+            out.r[26] = sl1;   // lane 6 -> x 7
+            out.r[27] = sr1;   // lane 4 -> x 5
+        }
+        if (g.leader()) {
+            const float v[24] = {red, red_init, bc0, bc_lin, bc_id, any_t ? 1.f : 0.f, all_t ? 1.f : 0.f, none_t ? 1.f : 0.f,
+                                 any_f ? 1.f : 0.f, jred, jred_i, jany ? 1.f : 0.f, jall ? 1.f : 0.f, jnone ? 1.f : 0.f, red_max, px,
+                                 sel, sl, sr, red_min, jmin, static_cast<float>(sg.get_local_linear_range()),
+                                 static_cast<float>(sg.get_group_range()[0]), static_cast<float>(sg.get_max_local_range()[0])};
+            for (std::size_t k = 0; k < 24; ++k) out.r[k] = v[k];
+        }
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+
+    void declare() override { this->declare_output(outputs.r, "r", 32 + 6 * n + 2 * M); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return this->create_launch_config(cs::range_desc(cs::range<2>{W0, W1}), cs::wg_desc(cs::range<2>{W0, W1}),
+                                          cs::local_desc(cs::range<1>{2 * M}));
+    }
+};
+
 CS_COMPONENTS_END
+
+/// The closed forms of FamStage's results.
+template <std::size_t W0, std::size_t W1, std::size_t M>
+void group_functions_case() {
+    constexpr std::size_t n     = W0 * W1;
+    cs::Model             model = make([](cs::System& s) { s.add_subsystem(cs::subsystem_desc<FamStage<W0, W1, M>>("a")); });
+    model.setup();
+    model.run();
+    model.wait();
+    const std::vector<float> r = out_of(model, "a.r");
+    CHECK(r.size() == 32 + 6 * n + 2 * M, "size %zu", r.size());
+    if (r.size() != 32 + 6 * n + 2 * M) return;
+    const auto f = [](std::size_t v) { return static_cast<float>(v); };
+    const std::size_t lanes0 = n < 64 ? n : 64;
+    const float want[24] = {f(n * (n + 1) / 2), f(n * (n + 1) / 2 + 10), 1, 6, f(W1 + 3), 1, 1, 1, 0, f(M * (M + 1) / 2),
+                            f(M * (M + 1) / 2 + 1), 1, 1, 1, f(n), 1, 1, 1, 1, 1, 1, f(lanes0), f((n + 63) / 64), 64};
+    for (std::size_t k = 0; k < 24; ++k) CHECK(r[k] == want[k], "%zux%zu group-wide result %zu: %g, want %g", W0, W1, k, r[k], want[k]);
+    const float shuf[4] = {5, 4, 7, 5};
+    for (std::size_t k = 0; k < 4; ++k) CHECK(r[24 + k] == shuf[k], "%zux%zu shuffle %zu: %g, want %g", W0, W1, k, r[24 + k], shuf[k]);
+    for (std::size_t t = 0; t < n; ++t) {
+        const float ex = f(t * (t + 1) / 2), in = f((t + 1) * (t + 2) / 2);
+        const float jex = t < M ? ex : 0.0f, jin = t < M ? in : 0.0f;
+        const float got[6] = {r[32 + t * 6], r[32 + t * 6 + 1], r[32 + t * 6 + 2], r[32 + t * 6 + 3], r[32 + t * 6 + 4], r[32 + t * 6 + 5]};
+        const float exp[6] = {ex, in, ex + 100, in + 100, jex, jin};
+        for (int k = 0; k < 6; ++k) CHECK(got[k] == exp[k], "%zux%zu work-item %zu scan %d: %g, want %g", W0, W1, t, k, got[k], exp[k]);
+    }
+    for (std::size_t i = 0; i < M; ++i) {
+        CHECK(r[32 + 6 * n + i] == f(i * (i + 1) / 2), "%zux%zu joint exclusive %zu: %g", W0, W1, i, r[32 + 6 * n + i]);
+        CHECK(r[32 + 6 * n + M + i] == f((i + 1) * (i + 2) / 2), "%zux%zu joint inclusive %zu: %g", W0, W1, i, r[32 + 6 * n + M + i]);
+    }
+}
 
 using TagStage   = GStage<kTag>;
 using RangeStage = GStage<kRange>;
@@ -186,6 +313,11 @@ int main() {
     int cus = 0;
     cs_hip_device_info(0, nullptr, 0, &cus, nullptr, nullptr);
     const std::size_t big_wg = 1024, big_groups = 4 * static_cast<std::size_t>(std::max(cus, 1));
+
+    std::printf("group functions, one group of 4 x 4 and of 4 x 25 (a partly populated second wave)\n");
+    group_functions_case<4, 4, 16>();
+    group_functions_case<4, 25, 307>();
+    group_functions_case<8, 32, 1000>();
 
     std::printf("two segments, each stage sees its own segment's group count\n");
     {
