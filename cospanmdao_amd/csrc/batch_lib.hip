@@ -1,9 +1,10 @@
 // cospanmdao_amd/csrc/batch_lib.hip — the batched ragged GEMVER model behind the C boundary, with its autotuner.
 //
-// A handle owns a cs::Model whose root is workloads/batch_systems.hpp's BatchedGemver. Runs after the first two are
-// replays of one captured graph (plan, scan, persistent kernel). The autotuner times every candidate (strip rows R,
-// pass-2 rows RT, persistent work-groups G) on the model itself and keeps the fastest; results are cached per device and
-// problem in a small text file so later processes skip the search.
+// A handle owns a cs::Model whose root is workloads/batch_systems.hpp's BatchedGemver, in its phased form (plan, scan,
+// pass 1, column reductions, pass 2: five launches) unless cs_batch_set_mode asks for the persistent one (plan, scan, one
+// work-queue kernel). Runs after the first two are replays of one captured graph. The autotuner times every candidate
+// (strip rows R, pass-2 rows RT and, persistent form only, work-groups G) on the model itself and keeps the fastest;
+// results are cached per device, problem and form in a small text file so later processes skip the search.
 
 #include <hip/hip_runtime.h>
 
@@ -95,6 +96,7 @@ struct cs_batch_s {
     std::shared_ptr<batch::Layout> layout;
     std::shared_ptr<batch::Knobs>  knobs = std::make_shared<batch::Knobs>();
     std::vector<std::uint32_t>     params{64, 8, 1024};
+    bool                           persistent = false;
     float                          alpha = 1.5f, beta = 1.2f;
     std::uint32_t                  seed = 0;
     std::unique_ptr<cs::Model>     model;
@@ -105,8 +107,9 @@ namespace {
 cs::Model& model(cs_batch_s& g) {
     if (!g.model) {
         g.knobs->groups = static_cast<int>(g.params[2]);
-        g.model         = std::make_unique<cs::Model>(
-            cs::create_model<batch::BatchedGemver>(g.layout, g.knobs, g.params, g.alpha, g.beta, g.seed, a_init(g.seed * 16u + 1u)));
+        batch::count_units(g.layout->sizes.data(), g.layout->sizes.size(), g.params[0], g.params[1], g.knobs->units);
+        g.model = std::make_unique<cs::Model>(cs::create_model<batch::BatchedGemver>(g.layout, g.knobs, g.params, g.alpha, g.beta, g.seed,
+                                                                                     a_init(g.seed * 16u + 1u), !g.persistent));
         g.model->setup();
     }
     return *g.model;
@@ -118,6 +121,7 @@ void set_params(cs_batch_s& g, std::uint32_t R, std::uint32_t RT, std::uint32_t 
     if (G < 1 || G > 65536) throw std::invalid_argument("batch: the persistent grid is 1 .. 65536 work-groups");
     g.params = {R, RT, G};
     g.knobs->groups = static_cast<int>(G);
+    batch::count_units(g.layout->sizes.data(), g.layout->sizes.size(), R, RT, g.knobs->units);
     if (g.model) g.model->update_input("params", cs::init::from_host(g.params));
 }
 
@@ -139,7 +143,7 @@ std::string cache_key(const cs_batch_s& g) {
     std::uint64_t h = 1469598103934665603ull;
     for (std::uint32_t n : g.layout->sizes) h = (h ^ n) * 1099511628211ull;
     std::ostringstream o;
-    o << name << ":cus" << cus << ":K" << g.layout->sizes.size() << ":sizes" << std::hex << h;
+    o << name << ":cus" << cus << ":K" << g.layout->sizes.size() << ":sizes" << std::hex << h << (g.persistent ? ":persistent" : ":phased");
     return o.str();
 }
 
@@ -192,6 +196,18 @@ int cs_batch_set_params(cs_batch g, uint32_t R, uint32_t RT, uint32_t groups) {
     return wl::guard([&] {
         if (!g) throw std::invalid_argument("cs_batch_set_params: null handle");
         set_params(*g, R, RT, groups);
+    });
+}
+
+int cs_batch_set_mode(cs_batch g, int persistent) {
+    return wl::guard([&] {
+        if (!g) throw std::invalid_argument("cs_batch_set_mode: null handle");
+        if (persistent != 0 && persistent != 1) throw std::invalid_argument("cs_batch_set_mode: mode is 0 (phased) or 1 (persistent)");
+        if ((persistent != 0) != g->persistent) {
+            if (g->model) g->model->wait();
+            g->model.reset();  // rebuilt in the other form at the next use
+            g->persistent = persistent != 0;
+        }
     });
 }
 
@@ -273,9 +289,10 @@ int cs_batch_autotune(cs_batch g, int reps, const char* cache_path, uint32_t* be
             cs::detail::hip_check(cs_hip_device_info(0, nullptr, 0, &cus, nullptr, nullptr), "device info");
             float         best = 1e30f;
             std::uint32_t bp[3] = {64, 8, static_cast<std::uint32_t>(4 * cus)};
+            const std::vector<std::uint32_t> gms = g->persistent ? std::vector<std::uint32_t>{2u, 4u, 8u} : std::vector<std::uint32_t>{4u};
             for (std::uint32_t R : {32u, 64u, 128u})
                 for (std::uint32_t RT : {4u, 8u})
-                    for (std::uint32_t gm : {2u, 4u, 8u}) {
+                    for (std::uint32_t gm : gms) {
                         set_params(*g, R, RT, gm * static_cast<std::uint32_t>(cus));
                         m.run();  // eager (new parameters), then a capture, then replays
                         m.run();

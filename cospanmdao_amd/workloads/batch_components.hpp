@@ -24,9 +24,12 @@
 //                  reductions are: every wait is on units with earlier tickets, held by work-groups that are running them
 //                  and never wait on later ones, so the queue drains without co-residency. The last work-group to leave
 //                  zeroes the counters (graph replays need no memset).
+//   RaggedPass1 / RaggedReduce / RaggedPass2 (the phased form) the same units as three plain launches, one work-group per
+//                  unit (work-group u of a phase runs that phase's unit u): no work queue and no waits between
+//                  work-groups; the launch boundaries order the phases.
 // The arithmetic of a tile is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
-// the reference components' f32 x for the same R bit for bit. R, RT (4 or 8) and G are run-time parameters (the
-// autotuner's knobs).
+// the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4 or 8) and G are run-time parameters
+// (the autotuner's knobs; G only for the persistent form).
 
 #include <cstdint>
 #include <memory>
@@ -45,10 +48,155 @@ inline constexpr int kMaxRT = 8;
 __host__ __device__ inline std::uint32_t ceil_div(std::uint32_t a, std::uint32_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline std::uint32_t ld_of(std::uint32_t n) { return (n + 3u) & ~3u; }
 
-/// Knobs shared between the model's owner (the autotuner) and the persistent component: read at every describe.
+/// Knobs shared between the model's owner (the autotuner) and the components that size their grids from them: read at
+/// every describe. `units` = the three phases' work-unit counts for the current R and RT (what the device plan computes).
 struct Knobs {
-    int groups = 1024;
+    int           groups   = 1024;
+    std::uint64_t units[3] = {1, 1, 1};
 };
+
+/// The phases' unit counts over the instances: pass-1 tiles, column-reduction blocks, pass-2 row groups.
+inline void count_units(const std::uint32_t* sizes, std::size_t K, std::uint32_t R, std::uint32_t RT, std::uint64_t (&u)[3]) {
+    u[0] = u[1] = u[2] = 0;
+    for (std::size_t k = 0; k < K; ++k) {
+        const std::uint32_t n = sizes[k], bl = ceil_div(n, kCols);
+        u[0] += static_cast<std::uint64_t>(ceil_div(n, R)) * bl;
+        u[1] += bl;
+        u[2] += ceil_div(n, RT);
+    }
+}
+
+/// The device scan's six offset columns (column c of instance k = scan[cK + k] - scan[cK]).
+struct Cols {
+    const std::uint32_t* s;
+    std::uint32_t        K;
+    __device__ std::uint32_t off(int c, std::uint32_t k) const { return s[c * K + k] - s[c * K]; }
+    __device__ std::uint32_t total(int c) const { return s[(c + 1) * K] - s[c * K]; }
+    /// the instance whose range of column c holds unit u: the largest k with off(c, k) <= u
+    __device__ std::uint32_t find(int c, std::uint32_t u) const {
+        std::uint32_t lo = 0, hi = K - 1;
+        while (lo < hi) {
+            const std::uint32_t mid = (lo + hi + 1) / 2;
+            if (off(c, mid) <= u)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
+};
+
+/// What the units read and write (raw device pointers; a phase leaves the arrays it does not touch null).
+struct Arrays {
+    const std::uint32_t* sizes = nullptr;
+    Cols                 cols{nullptr, 0};
+    std::uint32_t        R     = 0;
+    const float *        A = nullptr, *u1 = nullptr, *v1 = nullptr, *u2 = nullptr, *v2 = nullptr, *y = nullptr, *z = nullptr;
+    const float*         x_in  = nullptr;  ///< x as pass 2 reads it
+    const float*         P_in  = nullptr;  ///< P as the column reduction reads it
+    float *              P = nullptr, *x = nullptr, *w = nullptr;
+    float                alpha = 0.0f, beta = 0.0f;
+};
+
+// ---- the units ------------------------------------------------------------------------------------------------------
+
+/// pass-1 tile `t` of instance k: strip c = t / blocks, column block b = t % blocks.
+__device__ inline void unit_pass1(const Arrays& a, std::uint32_t k, std::uint32_t t, std::uint32_t tid) {
+    const std::uint32_t n = a.sizes[k], ld = ld_of(n), blocks = ceil_div(n, kCols), R = a.R;
+    const std::uint32_t c = t / blocks, b = t % blocks;
+    const std::uint32_t j0 = b * kCols + tid * 4;
+    if (j0 >= n) return;
+    const std::uint32_t i0 = c * R, i1 = i0 + R < n ? i0 + R : n;
+    const std::size_t   vo = a.cols.off(0, k);
+    const float*        A  = a.A + 4 * static_cast<std::size_t>(a.cols.off(1, k));
+    const float *       u1 = a.u1 + vo, *v1 = a.v1 + vo, *u2 = a.u2 + vo, *v2 = a.v2 + vo, *y = a.y + vo;
+    float*              P  = a.P + a.cols.off(2, k);
+    float               acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (j0 + 4 <= n) {
+        using F4      = float __attribute__((ext_vector_type(4)));
+        const F4 a1   = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
+        const F4 b1   = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
+        const float* ap = A + static_cast<std::size_t>(i0) * ld + j0;
+#pragma unroll 8
+        for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
+            const F4    r  = __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap));
+            const float yi = y[i], p1 = u1[i], p2 = u2[i];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[e] = fmaf(fmaf(p2, b1[e], fmaf(p1, a1[e], r[e])), yi, acc[e]);
+        }
+        for (int e = 0; e < 4; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
+    } else {
+        const std::uint32_t m = n - j0;
+        for (std::uint32_t i = i0; i < i1; ++i)
+            for (std::uint32_t e = 0; e < m; ++e)
+                acc[e] = fmaf(fmaf(u2[i], v2[j0 + e], fmaf(u1[i], v1[j0 + e], A[static_cast<std::size_t>(i) * ld + j0 + e])), y[i], acc[e]);
+        for (std::uint32_t e = 0; e < m; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
+    }
+}
+
+/// reduction unit b of instance k: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
+__device__ inline void unit_reduce(const Arrays& a, std::uint32_t k, std::uint32_t b, std::uint32_t tid) {
+    const std::uint32_t n = a.sizes[k], strips = ceil_div(n, a.R);
+    const std::size_t   vo = a.cols.off(0, k);
+    const float*        P  = a.P_in + a.cols.off(2, k);
+    for (std::uint32_t j = b * kCols + tid; j < n && j < (b + 1) * kCols; j += kWg) {
+        float acc = 0.0f;
+#pragma unroll 8
+        for (std::uint32_t c = 0; c < strips; ++c) acc += P[static_cast<std::size_t>(c) * n + j];
+        a.x[vo + j] = fmaf(a.beta, acc, a.z[vo + j]);
+    }
+}
+
+/// pass-2 unit g of instance k: rows [g RT, g RT + RT). Ends with a barrier (its LDS is reused by the next unit).
+template <int RT>
+__device__ inline void unit_pass2(const Arrays& a, std::uint32_t k, std::uint32_t g, std::uint32_t tid) {
+    const std::uint32_t n = a.sizes[k], ld = ld_of(n);
+    const std::uint32_t r0 = g * RT, nr = n - r0 < static_cast<std::uint32_t>(RT) ? n - r0 : static_cast<std::uint32_t>(RT);
+    const std::size_t   vo = a.cols.off(0, k);
+    const float*        A  = a.A + 4 * static_cast<std::size_t>(a.cols.off(1, k));
+    const float *       u1 = a.u1 + vo, *v1 = a.v1 + vo, *u2 = a.u2 + vo, *v2 = a.v2 + vo, *x = a.x_in + vo;
+    float               acc[RT], p1[RT], p2[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const std::uint32_t i = r0 + (static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u);
+        p1[r]                 = u1[i];
+        p2[r]                 = u2[i];
+        acc[r]                = 0.0f;
+    }
+    using F4         = float __attribute__((ext_vector_type(4)));
+    std::uint32_t j0 = tid * 4;
+    for (; j0 + 4 <= n; j0 += kCols) {
+        const F4 av = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
+        const F4 bv = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
+        const F4 xv = F4{x[j0], x[j0 + 1], x[j0 + 2], x[j0 + 3]};
+        F4       t[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+            t[r]                   = __builtin_nontemporal_load(reinterpret_cast<const F4*>(A + static_cast<std::size_t>(r0 + rr) * ld + j0));
+        }
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[r] = fmaf(fmaf(p2[r], bv[e], fmaf(p1[r], av[e], t[r][e])), xv[e], acc[r]);
+    }
+    if (j0 < n)
+        for (int r = 0; r < RT; ++r) {
+            const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+            for (std::uint32_t j = j0; j < n; ++j)
+                acc[r] = fmaf(fmaf(p2[r], v2[j], fmaf(p1[r], v1[j], A[static_cast<std::size_t>(r0 + rr) * ld + j])), x[j], acc[r]);
+        }
+    __shared__ float ws[kWg / 64][kMaxRT];
+    const int        wv = static_cast<int>(tid) / 64;
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const float v = cs::wave::reduce(acc[r], cs::plus<float>{});
+        if (cs::wave::lane_id() == 0) ws[wv][r] = v;
+    }
+    __syncthreads();
+    if (tid < nr) a.w[vo + r0 + tid] = a.alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
+    __syncthreads();
+}
 
 CS_COMPONENTS_BEGIN
 
@@ -135,29 +283,11 @@ public:
         cs::Output<DType::u32, 1> state;  ///< [ticket, finished, pass-1 done per instance, reductions done per instance]
     };
 
-    struct Cols {
-        const std::uint32_t* s;  // the scan
-        std::uint32_t        K;
-        __device__ std::uint32_t off(int c, std::uint32_t k) const { return s[c * K + k] - s[c * K]; }
-        __device__ std::uint32_t total(int c) const { return s[(c + 1) * K] - s[c * K]; }
-        /// the instance whose range of column c holds unit u: the largest k with off(c, k) <= u
-        __device__ std::uint32_t find(int c, std::uint32_t u) const {
-            std::uint32_t lo = 0, hi = K - 1;
-            while (lo < hi) {
-                const std::uint32_t mid = (lo + hi + 1) / 2;
-                if (off(c, mid) <= u)
-                    lo = mid;
-                else
-                    hi = mid - 1;
-            }
-            return lo;
-        }
-    };
-
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Cols          cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
-        const std::uint32_t K  = cols.K;
-        const std::uint32_t R  = in.params[std::size_t{0}], RT = in.params[std::size_t{1}];
+        const Arrays        a    = arrays(in, out);
+        const Cols&         cols = a.cols;
+        const std::uint32_t K    = cols.K;
+        const std::uint32_t RT   = in.params[std::size_t{1}];
         const std::uint32_t T1 = cols.total(3), TR = cols.total(4), T2 = cols.total(5), T = T1 + TR + T2;
         std::uint32_t*      st = cs::raw::data(out.state);
         std::uint32_t*      p1_done  = st + 2;
@@ -177,21 +307,21 @@ public:
             if (u >= T) break;
             if (u < T1) {
                 const std::uint32_t k = cols.find(3, u);
-                pass1(in, out, cols, k, u - cols.off(3, k), R, tid);
+                unit_pass1(a, k, u - cols.off(3, k), tid);
                 complete(&p1_done[k], tid);
             } else if (u < T1 + TR) {
                 const std::uint32_t k = cols.find(4, u - T1);
                 wait_for(&p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
-                reduce(in, out, cols, k, u - T1 - cols.off(4, k), tid);
+                unit_reduce(a, k, u - T1 - cols.off(4, k), tid);
                 complete(&red_done[k], tid);
             } else {
                 const std::uint32_t k = cols.find(5, u - T1 - TR);
                 wait_for(&red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
                 const std::uint32_t g = u - T1 - TR - cols.off(5, k);
                 if (RT == 4)
-                    pass2<4>(in, out, cols, k, g, tid);
+                    unit_pass2<4>(a, k, g, tid);
                 else
-                    pass2<8>(in, out, cols, k, g, tid);
+                    unit_pass2<8>(a, k, g, tid);
             }
         }
         // leaving: the last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
@@ -224,110 +354,26 @@ public:
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
 
-    /// pass-1 tile `t` of instance k: strip c = t / blocks, column block b = t % blocks.
-    __device__ static void pass1(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t t, std::uint32_t R,
-                                 std::uint32_t tid) {
-        const std::uint32_t n = in.sizes[std::size_t{k}], ld = ld_of(n), blocks = ceil_div(n, kCols);
-        const std::uint32_t c = t / blocks, b = t % blocks;
-        const std::uint32_t j0 = b * kCols + tid * 4;
-        if (j0 >= n) return;
-        const std::uint32_t i0 = c * R, i1 = i0 + R < n ? i0 + R : n;
-        const std::size_t   vo = cols.off(0, k);
-        const float*        A  = cs::raw::data(in.A) + 4 * static_cast<std::size_t>(cols.off(1, k));
-        const float *       u1 = cs::raw::data(in.u1) + vo, *v1 = cs::raw::data(in.v1) + vo, *u2 = cs::raw::data(in.u2) + vo;
-        const float *       v2 = cs::raw::data(in.v2) + vo, *y = cs::raw::data(in.y) + vo;
-        float*              P  = cs::raw::data(out.P) + cols.off(2, k);
-        float               acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (j0 + 4 <= n) {
-            using F4  = float __attribute__((ext_vector_type(4)));
-            const F4 a1 = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
-            const F4 b1 = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
-            const float* ap = A + static_cast<std::size_t>(i0) * ld + j0;
-#pragma unroll 8
-            for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
-                const F4    r  = __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap));
-                const float yi = y[i], p1 = u1[i], p2 = u2[i];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[e] = fmaf(fmaf(p2, b1[e], fmaf(p1, a1[e], r[e])), yi, acc[e]);
-            }
-            for (int e = 0; e < 4; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
-        } else {
-            const std::uint32_t m = n - j0;
-            for (std::uint32_t i = i0; i < i1; ++i)
-                for (std::uint32_t e = 0; e < m; ++e)
-                    acc[e] = fmaf(fmaf(u2[i], v2[j0 + e], fmaf(u1[i], v1[j0 + e], A[static_cast<std::size_t>(i) * ld + j0 + e])), y[i], acc[e]);
-            for (std::uint32_t e = 0; e < m; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
-        }
-    }
-
-    /// reduction unit b of instance k: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
-    __device__ static void reduce(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t b, std::uint32_t tid) {
-        const std::uint32_t n = in.sizes[std::size_t{k}];
-        const std::uint32_t R = in.params[std::size_t{0}], strips = ceil_div(n, R);
-        const std::size_t   vo = cols.off(0, k);
-        const float*        P  = cs::raw::data(out.P) + cols.off(2, k);
-        const float         beta = in.beta[std::size_t{0}];
-        for (std::uint32_t j = b * kCols + tid; j < n && j < (b + 1) * kCols; j += kWg) {
-            float acc = 0.0f;
-#pragma unroll 8
-            for (std::uint32_t c = 0; c < strips; ++c) acc += P[static_cast<std::size_t>(c) * n + j];
-            out.x[std::size_t{vo + j}] = fmaf(beta, acc, in.z[std::size_t{vo + j}]);
-        }
-    }
-
-    /// pass-2 unit g of instance k: rows [g RT, g RT + RT)
-    template <int RT>
-    __device__ static void pass2(const Inputs& in, const Outputs& out, const Cols& cols, std::uint32_t k, std::uint32_t g, std::uint32_t tid) {
-        const std::uint32_t n = in.sizes[std::size_t{k}], ld = ld_of(n);
-        const std::uint32_t r0 = g * RT, nr = n - r0 < static_cast<std::uint32_t>(RT) ? n - r0 : static_cast<std::uint32_t>(RT);
-        const std::size_t   vo = cols.off(0, k);
-        const float*        A  = cs::raw::data(in.A) + 4 * static_cast<std::size_t>(cols.off(1, k));
-        const float *       u1 = cs::raw::data(in.u1) + vo, *v1 = cs::raw::data(in.v1) + vo, *u2 = cs::raw::data(in.u2) + vo;
-        const float *       v2 = cs::raw::data(in.v2) + vo, *x = cs::raw::data(out.x) + vo;
-        float               acc[RT], p1[RT], p2[RT];
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-            const std::uint32_t i = r0 + (static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u);
-            p1[r]                 = u1[i];
-            p2[r]                 = u2[i];
-            acc[r]                = 0.0f;
-        }
-        using F4         = float __attribute__((ext_vector_type(4)));
-        std::uint32_t j0 = tid * 4;
-        for (; j0 + 4 <= n; j0 += kCols) {
-            const F4 a = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
-            const F4 b = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
-            const F4 xv = F4{x[j0], x[j0 + 1], x[j0 + 2], x[j0 + 3]};
-            F4       t[RT];
-#pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
-                t[r]                   = __builtin_nontemporal_load(reinterpret_cast<const F4*>(A + static_cast<std::size_t>(r0 + rr) * ld + j0));
-            }
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[r] = fmaf(fmaf(p2[r], b[e], fmaf(p1[r], a[e], t[r][e])), xv[e], acc[r]);
-        }
-        if (j0 < n)
-            for (int r = 0; r < RT; ++r) {
-                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
-                for (std::uint32_t j = j0; j < n; ++j)
-                    acc[r] = fmaf(fmaf(p2[r], v2[j], fmaf(p1[r], v1[j], A[static_cast<std::size_t>(r0 + rr) * ld + j])), x[j], acc[r]);
-            }
-        __shared__ float ws[kWg / 64][kMaxRT];
-        const int        wv = static_cast<int>(tid) / 64;
-#pragma unroll
-        for (int r = 0; r < RT; ++r) {
-            const float v = cs::wave::reduce(acc[r], cs::plus<float>{});
-            if (cs::wave::lane_id() == 0) ws[wv][r] = v;
-        }
-        __syncthreads();
-        if (tid < nr) {
-            const float alpha = in.alpha[std::size_t{0}];
-            out.w[std::size_t{vo + r0 + tid}] = alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
-        }
-        __syncthreads();  // ws is reused by the next unit
+    __device__ static Arrays arrays(const Inputs& in, const Outputs& out) {
+        Arrays a;
+        a.sizes = cs::raw::data(in.sizes);
+        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        a.R     = in.params[std::size_t{0}];
+        a.A     = cs::raw::data(in.A);
+        a.u1    = cs::raw::data(in.u1);
+        a.v1    = cs::raw::data(in.v1);
+        a.u2    = cs::raw::data(in.u2);
+        a.v2    = cs::raw::data(in.v2);
+        a.y     = cs::raw::data(in.y);
+        a.z     = cs::raw::data(in.z);
+        a.P     = cs::raw::data(out.P);
+        a.P_in  = cs::raw::data(out.P);
+        a.x     = cs::raw::data(out.x);
+        a.x_in  = cs::raw::data(out.x);
+        a.w     = cs::raw::data(out.w);
+        a.alpha = in.alpha[std::size_t{0}];
+        a.beta  = in.beta[std::size_t{0}];
+        return a;
     }
 
     Inputs  inputs;
@@ -358,6 +404,158 @@ public:
         return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->groups) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
     }
     explicit RaggedGemver(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
+
+private:
+    std::shared_ptr<Knobs> knobs_;
+};
+
+// ---- the phased form: one plain launch per phase, work-group u runs unit u -----------------------------------------
+
+/// Pass 1 of every instance: P (strip partials) from A and u1 v1 u2 v2 y.
+class RaggedPass1 : public cs::Component<RaggedPass1> {
+public:
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::f32, 1> A, u1, v1, u2, v2, y;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> P;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        Arrays a;
+        a.sizes = cs::raw::data(in.sizes);
+        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
+        if (u >= a.cols.total(3)) return;  // a grid larger than the device plan's count: nothing to do
+        a.R  = in.params[std::size_t{0}];
+        a.A  = cs::raw::data(in.A);
+        a.u1 = cs::raw::data(in.u1);
+        a.v1 = cs::raw::data(in.v1);
+        a.u2 = cs::raw::data(in.u2);
+        a.v2 = cs::raw::data(in.v2);
+        a.y  = cs::raw::data(in.y);
+        a.P  = cs::raw::data(out.P);
+        const std::uint32_t k = a.cols.find(3, u);
+        unit_pass1(a, k, u - a.cols.off(3, k), static_cast<std::uint32_t>(it.get_local_id(0)));
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.params, "params", 3);
+        declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
+                               {&inputs.y, "y"}})
+            declare_input(*w, name, cs::unspecified_shape<1>());
+        declare_output(outputs.P, "P", cs::unspecified_shape<1>());
+    }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[0]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+    explicit RaggedPass1(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
+
+private:
+    std::shared_ptr<Knobs> knobs_;
+};
+
+/// The column reductions of every instance: x = beta sum_c P[c] + z.
+class RaggedReduce : public cs::Component<RaggedReduce> {
+public:
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::f32, 1> P, z, beta;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> x;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        Arrays a;
+        a.sizes = cs::raw::data(in.sizes);
+        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
+        if (u >= a.cols.total(4)) return;
+        a.R    = in.params[std::size_t{0}];
+        a.P_in = cs::raw::data(in.P);
+        a.z    = cs::raw::data(in.z);
+        a.x    = cs::raw::data(out.x);
+        a.beta = in.beta[std::size_t{0}];
+        const std::uint32_t k = a.cols.find(4, u);
+        unit_reduce(a, k, u - a.cols.off(4, k), static_cast<std::uint32_t>(it.get_local_id(0)));
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.params, "params", 3);
+        declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.P, "P", cs::unspecified_shape<1>());
+        declare_input(inputs.z, "z", cs::unspecified_shape<1>());
+        declare_input(inputs.beta, "beta", 1);
+        declare_output(outputs.x, "x", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { set_shape(outputs.x, shape_of(inputs.z)[0]); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[1]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+    explicit RaggedReduce(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
+
+private:
+    std::shared_ptr<Knobs> knobs_;
+};
+
+/// Pass 2 of every instance: w = alpha Â x, RT rows per work-group.
+class RaggedPass2 : public cs::Component<RaggedPass2> {
+public:
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::f32, 1> A, u1, v1, u2, v2, x, alpha;
+    };
+    struct Outputs {
+        cs::Output<DType::f32, 1> w;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        Arrays a;
+        a.sizes = cs::raw::data(in.sizes);
+        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
+        if (u >= a.cols.total(5)) return;
+        a.A     = cs::raw::data(in.A);
+        a.u1    = cs::raw::data(in.u1);
+        a.v1    = cs::raw::data(in.v1);
+        a.u2    = cs::raw::data(in.u2);
+        a.v2    = cs::raw::data(in.v2);
+        a.x_in  = cs::raw::data(in.x);
+        a.w     = cs::raw::data(out.w);
+        a.alpha = in.alpha[std::size_t{0}];
+        const std::uint32_t k = a.cols.find(5, u), g = u - a.cols.off(5, k), tid = static_cast<std::uint32_t>(it.get_local_id(0));
+        if (in.params[std::size_t{1}] == 4)
+            unit_pass2<4>(a, k, g, tid);
+        else
+            unit_pass2<8>(a, k, g, tid);
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.params, "params", 3);
+        declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
+                               {&inputs.x, "x"}})
+            declare_input(*w, name, cs::unspecified_shape<1>());
+        declare_input(inputs.alpha, "alpha", 1);
+        declare_output(outputs.w, "w", cs::unspecified_shape<1>());
+    }
+    void prepare_forward() override { set_shape(outputs.w, shape_of(inputs.x)[0]); }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[2]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+    explicit RaggedPass2(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
 
 private:
     std::shared_ptr<Knobs> knobs_;

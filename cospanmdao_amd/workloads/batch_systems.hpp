@@ -4,7 +4,8 @@
 // cospanmdao_amd/workloads/batch_systems.hpp — the batched ragged GEMVER as a System
 // =============================================================================
 //
-// Leaves: plan (BatchPlan) -> scan (ColumnScan) -> gemver (RaggedGemver, persistent). Ports: inputs sizes, params
+// Leaves: plan (BatchPlan) -> scan (ColumnScan) -> gemver (RaggedGemver, persistent), or in the phased form
+// plan -> scan -> p1 (RaggedPass1) -> red (RaggedReduce) -> p2 (RaggedPass2). Ports (both forms): inputs sizes, params
 // {R, RT, G}, A (all instances back to back, instance k at 16-byte unit offs[K + k] - offs[K]), u1 v1 u2 v2 y z (back to
 // back, instance k at offs[k]), alpha, beta; outputs offs (the device scan, 6K + 1 entries), result ([total, flag] of the
 // scan), x, w.
@@ -52,26 +53,46 @@ public:
     static constexpr std::uint32_t kMinR = 32;  // the smallest strip the autotuner tries: P is sized for it
 
     BatchedGemver(std::shared_ptr<Layout> layout, std::shared_ptr<Knobs> knobs, std::vector<std::uint32_t> params, float alpha, float beta,
-                  std::uint32_t base_seed, AInit a_init)
+                  std::uint32_t base_seed, AInit a_init, bool phased = false)
         : layout_{std::move(layout)}, knobs_{std::move(knobs)}, params_{std::move(params)}, alpha_{alpha}, beta_{beta}, seed_{base_seed},
-          a_init_{std::move(a_init)} {}
+          a_init_{std::move(a_init)}, phased_{phased} {}
 
     void setup() override {
         add_subsystem(cs::subsystem_desc<BatchPlan>("plan"));
         add_subsystem(cs::subsystem_desc<ColumnScan>("scan"));
-        add_subsystem(cs::subsystem_desc<RaggedGemver>("gemver", knobs_));
         connect("plan.counts", "scan.counts");
-        connect("scan.offs", "gemver.offs");
-        tie_inputs<DType::u32, 1>({"plan.sizes", "gemver.sizes"}, "sizes");
-        tie_inputs<DType::u32, 1>({"plan.params", "gemver.params"}, "params", cs::shape(3));
-        for (const char* v : {"A", "u1", "v1", "u2", "v2", "y", "z", "alpha", "beta"})
-            promote<DType::f32, 1>(std::string{"gemver."} + v, v);
+        if (phased_) {
+            add_subsystem(cs::subsystem_desc<RaggedPass1>("p1", knobs_));
+            add_subsystem(cs::subsystem_desc<RaggedReduce>("red", knobs_));
+            add_subsystem(cs::subsystem_desc<RaggedPass2>("p2", knobs_));
+            for (const char* leaf : {"p1.offs", "red.offs", "p2.offs"}) connect("scan.offs", leaf);
+            connect("p1.P", "red.P");
+            connect("red.x", "p2.x");
+            tie_inputs<DType::u32, 1>({"plan.sizes", "p1.sizes", "red.sizes", "p2.sizes"}, "sizes");
+            tie_inputs<DType::u32, 1>({"plan.params", "p1.params", "red.params", "p2.params"}, "params", cs::shape(3));
+            for (const char* v : {"A", "u1", "v1", "u2", "v2"})
+                tie_inputs<DType::f32, 1>({std::string{"p1."} + v, std::string{"p2."} + v}, v);
+            promote<DType::f32, 1>("p1.y", "y");
+            promote<DType::f32, 1>("red.z", "z");
+            promote<DType::f32, 1>("p2.alpha", "alpha");
+            promote<DType::f32, 1>("red.beta", "beta");
+            promote<DType::f32, 1>("red.x", "x");
+            promote<DType::f32, 1>("p2.w", "w");
+            shape_hook("p1.P", cs::dim::fixed_at_run(layout_->p_max));
+        } else {
+            add_subsystem(cs::subsystem_desc<RaggedGemver>("gemver", knobs_));
+            connect("scan.offs", "gemver.offs");
+            tie_inputs<DType::u32, 1>({"plan.sizes", "gemver.sizes"}, "sizes");
+            tie_inputs<DType::u32, 1>({"plan.params", "gemver.params"}, "params", cs::shape(3));
+            for (const char* v : {"A", "u1", "v1", "u2", "v2", "y", "z", "alpha", "beta"})
+                promote<DType::f32, 1>(std::string{"gemver."} + v, v);
+            promote<DType::f32, 1>("gemver.x", "x");
+            promote<DType::f32, 1>("gemver.w", "w");
+            shape_hook("gemver.P", cs::dim::fixed_at_run(layout_->p_max));
+        }
         promote<DType::u32, 1>("scan.offs", "offs");
         promote<DType::u32, 1>("scan.result", "result");
         promote<DType::u32, 1>("plan.counts", "counts");
-        promote<DType::f32, 1>("gemver.x", "x");
-        promote<DType::f32, 1>("gemver.w", "w");
-        shape_hook("gemver.P", cs::dim::fixed_at_run(layout_->p_max));
 
         const std::uint32_t sb = seed_ * 16u;
         init_input("sizes", cs::init::from_host(layout_->sizes));
@@ -90,6 +111,7 @@ private:
     float                      alpha_, beta_;
     std::uint32_t              seed_;
     AInit                      a_init_;
+    bool                       phased_;
 };
 
 }  // namespace batch

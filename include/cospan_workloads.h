@@ -88,10 +88,11 @@ int cs_ns_destroy(cs_ns m);
  * cospanmdao/include/naga/cospan/fused_component.hpp:207-213); the offsets come from the column build it uses for
  * ragged shapes (shape_hook.hpp:215-348). `count` instances with N_k = round(lo (hi/lo)^U_k) (U_k from the hash (seed, k));
  * each x_k = beta Â_k^T y_k + z_k, w_k = alpha Â_k x_k as GEMVER above. One forward = a plan kernel, ONE device scan of
- * the per-instance counts, ONE persistent work-queue launch; runs from the second on replay one captured graph.
+ * the per-instance counts, then (phased form, the default) one launch per phase over every instance: pass 1, the column
+ * reductions, pass 2 -- or (persistent form) ONE work-queue launch; runs from the second on replay one captured graph.
  * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (6 x count), "offs" (6 x count + 1: the scan),
  * "result" ([total, flag] of the scan), "x", "w", "u1" .. "z" (instances back to back).
- * params: R (strip rows, >= 32), RT (pass-2 rows: 4 or 8), G (persistent work-groups).
+ * params: R (strip rows, >= 32), RT (pass-2 rows: 4 or 8), G (persistent work-groups; the phased form ignores it).
  */
 typedef struct cs_batch_s* cs_batch;
 
@@ -99,8 +100,11 @@ int cs_batch_create(cs_batch* b, uint32_t count, uint32_t lo, uint32_t hi, float
 int cs_batch_sizes(cs_batch b, uint32_t* sizes, size_t count);
 int cs_batch_set_params(cs_batch b, uint32_t R, uint32_t RT, uint32_t groups);
 int cs_batch_get_params(cs_batch b, uint32_t* params3);
-/* times every candidate (R in {32, 64, 128}, RT in {4, 8}, G in {2, 4, 8} x CUs) on this model and keeps the fastest;
- * cache_path (or NULL): a text file of "key R RT G" lines keyed by device and sizes, read first and appended to */
+/* 0: the phased form (default), 1: the persistent work-queue form; a change rebuilds the model at its next use */
+int cs_batch_set_mode(cs_batch b, int persistent);
+/* times every candidate (R in {32, 64, 128}, RT in {4, 8}, and for the persistent form G in {2, 4, 8} x CUs) on this
+ * model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
+ * read first and appended to */
 int cs_batch_autotune(cs_batch b, int reps, const char* cache_path, uint32_t* best3);
 int cs_batch_run(cs_batch b);
 int cs_batch_wait(cs_batch b);

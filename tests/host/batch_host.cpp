@@ -1,11 +1,12 @@
 // tests/host/batch_host.cpp — the batched ragged GEMVER model's host logic on the CPU (TEST INFRASTRUCTURE ONLY).
 //
 // Builds workloads/batch_systems.hpp's BatchedGemver through the real Model / System / Executor code, linked against the
-// CPU implementation of the C boundary in mock_hip.cpp, with CPU emulations of the model's two kernels (BatchPlan and the
-// persistent RaggedGemver) registered under their host stubs. It checks what the kernels receive (shapes, pointers,
-// parameters, the scan's offsets, the work-queue state zeroed at every start) and what the model computes (x bit for bit
-// against the oracle's strip partials + column reduction, w against the FP64 oracle), over eager runs, a capture, replays
-// and parameter changes. Prints "OK" and exits 0 when every check holds.
+// CPU implementation of the C boundary in mock_hip.cpp, with CPU emulations of the model's kernels (BatchPlan, the
+// persistent RaggedGemver, and the phased form's RaggedPass1 / RaggedReduce / RaggedPass2) registered under their host
+// stubs. It checks what the kernels receive (shapes, pointers, parameters, grids against the unit counts, the scan's
+// offsets, the work-queue state zeroed at every start) and what the model computes (x bit for bit against the oracle's
+// strip partials + column reduction, w against the FP64 oracle), over eager runs, a capture, replays and parameter
+// changes, in both forms. Prints "OK" and exits 0 when every check holds.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -20,6 +21,9 @@
 namespace cs = naga::cospan;
 using GAPlan = cs::detail::GroupArgs<batch::BatchPlan>;
 using GAGem  = cs::detail::GroupArgs<batch::RaggedGemver>;
+using GAP1   = cs::detail::GroupArgs<batch::RaggedPass1>;
+using GARed  = cs::detail::GroupArgs<batch::RaggedReduce>;
+using GAP2   = cs::detail::GroupArgs<batch::RaggedPass2>;
 
 static int g_failures = 0;
 #define CHECK(cond, ...)                                             \
@@ -58,30 +62,18 @@ static void emulate_plan(const void* bytes, const std::uint32_t grid[3], const s
     }
 }
 
-// ---- RaggedGemver: the persistent work queue, its units in ticket order ----------------------------------------------
-static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
-    ++g_gemver_launches;
-    const auto&         a = static_cast<const GAGem*>(bytes)->args.head;
-    const std::uint32_t G = static_cast<std::uint32_t>(a.units[2]);
-    CHECK(grid[0] == G && grid[1] == 1 && grid[2] == 1 && block[0] == batch::kWg, "gemver: grid %u %u %u block %u for G %u", grid[0],
-          grid[1], grid[2], block[0], G);
-    const std::uint32_t K = static_cast<std::uint32_t>(a.in.sizes.size());
-    CHECK(a.out.state.size() == 2 + 2 * K, "gemver: state holds %zu words, not %u", a.out.state.size(), 2 + 2 * K);
-    CHECK(a.in.offs.size() == 6 * K + 1, "gemver: offs holds %zu words, not %u", a.in.offs.size(), 6 * K + 1);
-    std::uint32_t* st = cs::raw::data(a.out.state);
-    bool           zero = true;
-    for (std::uint32_t w = 0; w < 2 + 2 * K; ++w) zero = zero && st[w] == 0;
-    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u)",
-          (unsigned long long)g_gemver_launches, st[0], st[1]);
-    if (!zero) return;
+// ---- the units, as the device runs them (same fma order) -------------------------------------------------------------
+struct Emu {
+    const std::uint32_t *s = nullptr, *sz = nullptr;
+    std::uint32_t        K = 0, R = 0, RT = 0;
+    const float *        A = nullptr, *U1 = nullptr, *V1 = nullptr, *U2 = nullptr, *V2 = nullptr, *Y = nullptr, *Z = nullptr;
+    const float *        Pin = nullptr, *Xin = nullptr;
+    float *              P = nullptr, *X = nullptr, *W = nullptr;
+    float                alpha = 0.0f, beta = 0.0f;
 
-    const std::uint32_t* s  = cs::raw::data(a.in.offs);
-    const std::uint32_t* sz = cs::raw::data(a.in.sizes);
-    const std::uint32_t* pr = cs::raw::data(a.in.params);
-    const std::uint32_t  R = pr[0], RT = pr[1];
-    auto off   = [&](int c, std::uint32_t k) { return s[c * K + k] - s[c * K]; };
-    auto total = [&](int c) { return s[(c + 1) * K] - s[c * K]; };
-    auto find  = [&](int c, std::uint32_t u) {
+    std::uint32_t off(int c, std::uint32_t k) const { return s[c * K + k] - s[c * K]; }
+    std::uint32_t total(int c) const { return s[(c + 1) * K] - s[c * K]; }
+    std::uint32_t find(int c, std::uint32_t u) const {
         std::uint32_t lo = 0, hi = K - 1;
         while (lo < hi) {
             const std::uint32_t mid = (lo + hi + 1) / 2;
@@ -91,59 +83,148 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
                 hi = mid - 1;
         }
         return lo;
-    };
-    const std::uint32_t T1 = total(3), TR = total(4), T2 = total(5);
-    CHECK(std::size_t(s[6 * K]) <= (std::size_t(1) << 31), "gemver: the scan total %u is implausible", s[6 * K]);
-    const float *A = cs::raw::data(a.in.A), *U1 = cs::raw::data(a.in.u1), *V1 = cs::raw::data(a.in.v1), *U2 = cs::raw::data(a.in.u2);
-    const float *V2 = cs::raw::data(a.in.v2), *Y = cs::raw::data(a.in.y), *Z = cs::raw::data(a.in.z);
-    float *P = cs::raw::data(a.out.P), *X = cs::raw::data(a.out.x), *W = cs::raw::data(a.out.w);
-    const float alpha = cs::raw::data(a.in.alpha)[0], beta = cs::raw::data(a.in.beta)[0];
-    CHECK(a.in.A.size() >= 4ull * total(1), "gemver: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * total(1));
-    CHECK(a.out.P.size() >= total(2), "gemver: P holds %zu floats, the scan asks %u", a.out.P.size(), total(2));
-    CHECK(a.out.x.size() >= total(0) && a.in.u1.size() >= total(0), "gemver: vectors shorter than the scan's total");
+    }
+    /// pass-1 unit u (global over the instances); returns its instance
+    std::uint32_t pass1(std::uint32_t u) const {
+        const std::uint32_t k = find(3, u), t = u - off(3, k), n = sz[k], ld = batch::ld_of(n);
+        const std::uint32_t blocks = (n + batch::kCols - 1) / batch::kCols, c = t / blocks, b = t % blocks;
+        const std::size_t   vo = off(0, k);
+        const float*        Ak = A + 4ull * off(1, k);
+        float*              Pk = P + off(2, k);
+        const std::uint32_t i0 = c * R, i1 = std::min(i0 + R, n);
+        for (std::uint32_t j = b * batch::kCols; j < std::min<std::uint32_t>(n, (b + 1) * batch::kCols); ++j) {
+            float acc = 0.0f;
+            for (std::uint32_t i = i0; i < i1; ++i)
+                acc = std::fma(std::fma(U2[vo + i], V2[vo + j], std::fma(U1[vo + i], V1[vo + j], Ak[std::size_t(i) * ld + j])), Y[vo + i], acc);
+            Pk[std::size_t(c) * n + j] = acc;
+        }
+        return k;
+    }
+    std::uint32_t reduce(std::uint32_t u) const {
+        const std::uint32_t k = find(4, u), b = u - off(4, k), n = sz[k];
+        const std::uint32_t strips = (n + R - 1) / R;
+        const std::size_t   vo     = off(0, k);
+        const float*        Pk     = Pin + off(2, k);
+        for (std::uint32_t j = b * batch::kCols; j < std::min<std::uint32_t>(n, (b + 1) * batch::kCols); ++j) {
+            float acc = 0.0f;
+            for (std::uint32_t c = 0; c < strips; ++c) acc += Pk[std::size_t(c) * n + j];
+            X[vo + j] = std::fma(beta, acc, Z[vo + j]);
+        }
+        return k;
+    }
+    std::uint32_t pass2(std::uint32_t u) const {
+        const std::uint32_t k = find(5, u), g = u - off(5, k), n = sz[k], ld = batch::ld_of(n);
+        const std::size_t   vo = off(0, k);
+        const float*        Ak = A + 4ull * off(1, k);
+        for (std::uint32_t r = g * RT; r < std::min(n, g * RT + RT); ++r) {
+            double acc = 0.0;
+            for (std::uint32_t j = 0; j < n; ++j)
+                acc += (double(Ak[std::size_t(r) * ld + j]) + double(U1[vo + r]) * V1[vo + j] + double(U2[vo + r]) * V2[vo + j]) * Xin[vo + j];
+            W[vo + r] = static_cast<float>(alpha * acc);
+        }
+        return k;
+    }
+};
+
+/// The fields every ragged kernel reads: sizes, params, the scan (checked against its expected length).
+template <class In>
+static Emu emu_common(const In& in, const char* who) {
+    Emu e;
+    e.K  = static_cast<std::uint32_t>(in.sizes.size());
+    e.s  = cs::raw::data(in.offs);
+    e.sz = cs::raw::data(in.sizes);
+    e.R  = cs::raw::data(in.params)[0];
+    e.RT = cs::raw::data(in.params)[1];
+    CHECK(in.offs.size() == 6 * e.K + 1, "%s: offs holds %zu words, not %u", who, in.offs.size(), 6 * e.K + 1);
+    CHECK(std::size_t(e.s[6 * e.K]) <= (std::size_t(1) << 31), "%s: the scan total %u is implausible", who, e.s[6 * e.K]);
+    return e;
+}
+
+// ---- RaggedGemver: the persistent work queue, its units in ticket order ----------------------------------------------
+static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    ++g_gemver_launches;
+    const auto&         a = static_cast<const GAGem*>(bytes)->args.head;
+    const std::uint32_t G = static_cast<std::uint32_t>(a.units[2]);
+    CHECK(grid[0] == G && grid[1] == 1 && grid[2] == 1 && block[0] == batch::kWg, "gemver: grid %u %u %u block %u for G %u", grid[0],
+          grid[1], grid[2], block[0], G);
+    const std::uint32_t K = static_cast<std::uint32_t>(a.in.sizes.size());
+    CHECK(a.out.state.size() == 2 + 2 * K, "gemver: state holds %zu words, not %u", a.out.state.size(), 2 + 2 * K);
+    std::uint32_t* st = cs::raw::data(a.out.state);
+    bool           zero = true;
+    for (std::uint32_t w = 0; w < 2 + 2 * K; ++w) zero = zero && st[w] == 0;
+    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u)",
+          (unsigned long long)g_gemver_launches, st[0], st[1]);
+    if (!zero) return;
+
+    Emu e = emu_common(a.in, "gemver");
+    e.A = cs::raw::data(a.in.A), e.U1 = cs::raw::data(a.in.u1), e.V1 = cs::raw::data(a.in.v1), e.U2 = cs::raw::data(a.in.u2);
+    e.V2 = cs::raw::data(a.in.v2), e.Y = cs::raw::data(a.in.y), e.Z = cs::raw::data(a.in.z);
+    e.P = cs::raw::data(a.out.P), e.X = cs::raw::data(a.out.x), e.W = cs::raw::data(a.out.w);
+    e.Pin = e.P, e.Xin = e.X;
+    e.alpha = cs::raw::data(a.in.alpha)[0], e.beta = cs::raw::data(a.in.beta)[0];
+    const std::uint32_t T1 = e.total(3), TR = e.total(4), T2 = e.total(5);
+    CHECK(a.in.A.size() >= 4ull * e.total(1), "gemver: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
+    CHECK(a.out.P.size() >= e.total(2), "gemver: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
+    CHECK(a.out.x.size() >= e.total(0) && a.in.u1.size() >= e.total(0), "gemver: vectors shorter than the scan's total");
     std::vector<std::uint32_t> p1(K, 0), red(K, 0);
     for (std::uint32_t u = 0; u < T1 + TR + T2; ++u) {
         if (u < T1) {
-            const std::uint32_t k = find(3, u), t = u - off(3, k), n = sz[k], ld = batch::ld_of(n);
-            const std::uint32_t blocks = (n + batch::kCols - 1) / batch::kCols, c = t / blocks, b = t % blocks;
-            const std::size_t   vo = off(0, k);
-            const float*        Ak = A + 4ull * off(1, k);
-            float*              Pk = P + off(2, k);
-            const std::uint32_t i0 = c * R, i1 = std::min(i0 + R, n);
-            for (std::uint32_t j = b * batch::kCols; j < std::min<std::uint32_t>(n, (b + 1) * batch::kCols); ++j) {
-                float acc = 0.0f;
-                for (std::uint32_t i = i0; i < i1; ++i)
-                    acc = std::fma(std::fma(U2[vo + i], V2[vo + j], std::fma(U1[vo + i], V1[vo + j], Ak[std::size_t(i) * ld + j])), Y[vo + i], acc);
-                Pk[std::size_t(c) * n + j] = acc;
-            }
-            ++p1[k];
+            ++p1[e.pass1(u)];
         } else if (u < T1 + TR) {
-            const std::uint32_t k = find(4, u - T1), b = u - T1 - off(4, k), n = sz[k];
-            CHECK(p1[k] == s[3 * K + k + 1] - s[3 * K + k], "gemver: reduction of instance %u before its pass-1 tiles", k);
-            const std::uint32_t strips = (n + R - 1) / R;
-            const std::size_t   vo     = off(0, k);
-            const float*        Pk     = P + off(2, k);
-            for (std::uint32_t j = b * batch::kCols; j < std::min<std::uint32_t>(n, (b + 1) * batch::kCols); ++j) {
-                float acc = 0.0f;
-                for (std::uint32_t c = 0; c < strips; ++c) acc += Pk[std::size_t(c) * n + j];
-                X[vo + j] = std::fma(beta, acc, Z[vo + j]);
-            }
-            ++red[k];
+            const std::uint32_t k = e.find(4, u - T1);
+            CHECK(p1[k] == e.s[3 * K + k + 1] - e.s[3 * K + k], "gemver: reduction of instance %u before its pass-1 tiles", k);
+            ++red[e.reduce(u - T1)];
         } else {
-            const std::uint32_t k = find(5, u - T1 - TR), g = u - T1 - TR - off(5, k), n = sz[k], ld = batch::ld_of(n);
-            CHECK(red[k] == s[4 * K + k + 1] - s[4 * K + k], "gemver: pass 2 of instance %u before its reductions", k);
-            const std::size_t vo = off(0, k);
-            const float*      Ak = A + 4ull * off(1, k);
-            for (std::uint32_t r = g * RT; r < std::min(n, g * RT + RT); ++r) {
-                double acc = 0.0;
-                for (std::uint32_t j = 0; j < n; ++j)
-                    acc += (double(Ak[std::size_t(r) * ld + j]) + double(U1[vo + r]) * V1[vo + j] + double(U2[vo + r]) * V2[vo + j]) * X[vo + j];
-                W[vo + r] = static_cast<float>(alpha * acc);
-            }
+            const std::uint32_t k = e.find(5, u - T1 - TR);
+            CHECK(red[k] == e.s[4 * K + k + 1] - e.s[4 * K + k], "gemver: pass 2 of instance %u before its reductions", k);
+            e.pass2(u - T1 - TR);
         }
     }
     // the last work-group to leave zeroes the state (it was zero, and the queue leaves it so)
     std::memset(st, 0, (2 + 2 * K) * 4);
+}
+
+// ---- the phased form: one launch per phase, work-group u runs unit u; the grid must cover the device plan's count ----
+static std::uint64_t g_phase_launches[3] = {0, 0, 0};
+
+template <class GA>
+static void check_phase_grid(const GA& a, const std::uint32_t grid[3], const std::uint32_t block[3], std::uint32_t want, const char* who) {
+    CHECK(grid[0] == a.units[2] && grid[1] == 1 && grid[2] == 1 && block[0] == batch::kWg, "%s: grid %u %u %u block %u for %zu groups", who,
+          grid[0], grid[1], grid[2], block[0], (size_t)a.units[2]);
+    CHECK(a.units[2] == want, "%s: %zu work-groups for %u units", who, (size_t)a.units[2], want);
+}
+
+static void emulate_pass1(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    ++g_phase_launches[0];
+    const auto& a = static_cast<const GAP1*>(bytes)->args.head;
+    Emu         e = emu_common(a.in, "pass 1");
+    e.A = cs::raw::data(a.in.A), e.U1 = cs::raw::data(a.in.u1), e.V1 = cs::raw::data(a.in.v1), e.U2 = cs::raw::data(a.in.u2);
+    e.V2 = cs::raw::data(a.in.v2), e.Y = cs::raw::data(a.in.y), e.P = cs::raw::data(a.out.P);
+    check_phase_grid(a, grid, block, e.total(3), "pass 1");
+    CHECK(a.out.P.size() >= e.total(2), "pass 1: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
+    CHECK(a.in.A.size() >= 4ull * e.total(1), "pass 1: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(3)); ++u) e.pass1(u);
+}
+
+static void emulate_reduce(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    ++g_phase_launches[1];
+    const auto& a = static_cast<const GARed*>(bytes)->args.head;
+    Emu         e = emu_common(a.in, "reduce");
+    e.Pin = cs::raw::data(a.in.P), e.Z = cs::raw::data(a.in.z), e.X = cs::raw::data(a.out.x), e.beta = cs::raw::data(a.in.beta)[0];
+    check_phase_grid(a, grid, block, e.total(4), "reduce");
+    CHECK(a.out.x.size() >= e.total(0) && a.in.z.size() >= e.total(0), "reduce: vectors shorter than the scan's total");
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(4)); ++u) e.reduce(u);
+}
+
+static void emulate_pass2(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    ++g_phase_launches[2];
+    const auto& a = static_cast<const GAP2*>(bytes)->args.head;
+    Emu         e = emu_common(a.in, "pass 2");
+    e.A = cs::raw::data(a.in.A), e.U1 = cs::raw::data(a.in.u1), e.V1 = cs::raw::data(a.in.v1), e.U2 = cs::raw::data(a.in.u2);
+    e.V2 = cs::raw::data(a.in.v2), e.Xin = cs::raw::data(a.in.x), e.W = cs::raw::data(a.out.w), e.alpha = cs::raw::data(a.in.alpha)[0];
+    check_phase_grid(a, grid, block, e.total(5), "pass 2");
+    CHECK(a.out.w.size() >= e.total(0) && a.in.x.size() >= e.total(0), "pass 2: vectors shorter than the scan's total");
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(5)); ++u) e.pass2(u);
 }
 
 // ---- inputs: instance k's A from one flat hash stream, written on the host into the mock's device memory ------------
@@ -200,11 +281,9 @@ static void check_outputs(cs::Model& m, const std::vector<std::uint32_t>& sizes,
     std::printf("%s: %zu instances, worst w rel-L2 %.2e\n", what, sizes.size(), worst);
 }
 
-int main() {
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::index_kernel<batch::BatchPlan>), sizeof(GAPlan), emulate_plan);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedGemver>), sizeof(GAGem),
-                          emulate_gemver);
-
+/// One form of the model through eager runs, a capture, replays and a parameter change.
+static void run_form(bool phased) {
+    const char*         form  = phased ? "phased" : "persistent";
     const std::uint32_t count = 24, seed = 1;
     std::vector<std::uint32_t> sizes(count);
     or_ragged_sizes(seed, count, 16, 700, sizes.data());
@@ -213,26 +292,46 @@ int main() {
     auto        knobs  = std::make_shared<batch::Knobs>();
     std::vector<std::uint32_t> params{32, 4, 3};
     knobs->groups = 3;
-    cs::Model m   = cs::create_model<batch::BatchedGemver>(layout, knobs, params, alpha, beta, seed, host_a_init(seed * 16 + 1));
+    batch::count_units(sizes.data(), count, 32, 4, knobs->units);
+    cs::Model m = cs::create_model<batch::BatchedGemver>(layout, knobs, params, alpha, beta, seed, host_a_init(seed * 16 + 1), phased);
     m.setup();
+    char what[160];
 
     m.run();
-    check_outputs(m, sizes, 32, seed, alpha, beta, "eager run (R 32, RT 4, G 3)");
+    std::snprintf(what, sizeof what, "%s: eager run (R 32, RT 4, G 3)", form);
+    check_outputs(m, sizes, 32, seed, alpha, beta, what);
     for (int i = 0; i < 3; ++i) m.run();  // the second identical run is captured, then replays
-    check_outputs(m, sizes, 32, seed, alpha, beta, "capture + replays");
+    std::snprintf(what, sizeof what, "%s: capture + replays", form);
+    check_outputs(m, sizes, 32, seed, alpha, beta, what);
     const cs::ModelStats st1 = m.stats();
-    CHECK(st1.captures >= 1 && st1.replays >= 2, "captures %llu replays %llu", (unsigned long long)st1.captures,
+    CHECK(st1.captures >= 1 && st1.replays >= 2, "%s: captures %llu replays %llu", form, (unsigned long long)st1.captures,
           (unsigned long long)st1.replays);
 
     params        = {64, 8, 17};  // new parameters: new counts, a new grid
     knobs->groups = 17;
+    batch::count_units(sizes.data(), count, 64, 8, knobs->units);
     m.update_input("params", cs::init::from_host(params));
     for (int i = 0; i < 3; ++i) m.run();
-    check_outputs(m, sizes, 64, seed, alpha, beta, "after a parameter change (R 64, RT 8, G 17)");
+    std::snprintf(what, sizeof what, "%s: after a parameter change (R 64, RT 8, G 17)", form);
+    check_outputs(m, sizes, 64, seed, alpha, beta, what);
+}
+
+int main() {
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::index_kernel<batch::BatchPlan>), sizeof(GAPlan), emulate_plan);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedGemver>), sizeof(GAGem),
+                          emulate_gemver);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedPass1>), sizeof(GAP1), emulate_pass1);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedReduce>), sizeof(GARed),
+                          emulate_reduce);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedPass2>), sizeof(GAP2), emulate_pass2);
+
+    run_form(false);
+    run_form(true);
 
     const std::string f = mock::failure();
     CHECK(f.empty(), "mock: %s", f.c_str());
     CHECK(g_gemver_launches >= 4, "the persistent kernel ran %llu times", (unsigned long long)g_gemver_launches);
+    for (int p = 0; p < 3; ++p) CHECK(g_phase_launches[p] >= 4, "phase %d ran %llu times", p, (unsigned long long)g_phase_launches[p]);
     if (g_failures) {
         std::printf("%d failure(s)\n", g_failures);
         return 1;

@@ -1,5 +1,5 @@
 """The batched ragged GEMVER on the device: the offsets scan bit for bit, every instance's x and w against the oracle, the
-persistent launch under graph replay, the autotuner."""
+phased form (one launch per phase) and the persistent form (one work-queue launch) under graph replay, the autotuner."""
 import ctypes as C
 import os
 
@@ -9,24 +9,26 @@ import pytest
 import oracle as O
 from cospanmdao_amd import check, workloads
 
-pytestmark = [
-    pytest.mark.gpu,
-    # The persistent kernel hung on the device twice: first from a compiled loop nest in which the leader lane left the
-    # work loop to fetch the next ticket while the other lanes re-read the previous one (fixed: a barrier at the loop top,
-    # checked in the ISA), then because the library had not been rebuilt with that fix (the Makefile did not list the
-    # workload headers; fixed). The fixed kernel has not run on a GPU yet, so these tests stay skipped until it has; the
-    # model's host logic is checked on the CPU by tests/test_host_logic.py.
-    pytest.mark.skip(reason="persistent batched kernel: fix not yet verified on a GPU (see the comment above)"),
-]
+pytestmark = pytest.mark.gpu
+
+# The persistent kernel hung on the device twice: first from a compiled loop nest in which the leader lane left the work
+# loop to fetch the next ticket while the other lanes re-read the previous one (fixed: a barrier at the loop top, checked
+# in the ISA), then because the library had not been rebuilt with that fix (the Makefile did not list the workload
+# headers; fixed). The fixed kernel has not run on a GPU yet, so its tests stay skipped until it has; the model's host
+# logic (both forms) is checked on the CPU by tests/test_host_logic.py. The phased form has no work queue and no waits
+# between work-groups.
+PERSISTENT = pytest.mark.skip(reason="persistent batched kernel: fix not yet verified on a GPU (see the comment above)")
+PHASED, PERSISTENT_MODE = 0, 1
 
 ALPHA, BETA = 1.5, 1.2
 TOL_L2 = 1e-5
 
 
 class Batch:
-    def __init__(self, count, lo, hi, seed=0):
+    def __init__(self, count, lo, hi, seed=0, mode=PHASED):
         self.h = C.c_void_p()
         check(workloads.cs_batch_create(C.byref(self.h), count, lo, hi, ALPHA, BETA, seed))
+        check(workloads.cs_batch_set_mode(self.h, mode))
         self.count, self.seed = count, seed
         self.sizes = np.empty(count, np.uint32)
         check(workloads.cs_batch_sizes(self.h, self.sizes.ctypes.data_as(C.POINTER(C.c_uint32)), count))
@@ -105,10 +107,14 @@ def check_instances(b, which, R):
     return worst
 
 
+MODES = [pytest.param(PHASED, id="phased"), pytest.param(PERSISTENT_MODE, id="persistent", marks=PERSISTENT)]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("R,RT,G", [(32, 4, 3), (32, 8, 64), (64, 8, 17), (128, 4, 512)])
-def test_small_batch(R, RT, G):
-    print("batch small", R, RT, G, flush=True)
-    b = Batch(24, 16, 700, seed=1)
+def test_small_batch(R, RT, G, mode):
+    print("batch small", R, RT, G, "mode", mode, flush=True)
+    b = Batch(24, 16, 700, seed=1, mode=mode)
     try:
         sizes = np.empty(24, np.uint32)
         O.lib().or_ragged_sizes(1, 24, 16, 700, O.ptr(sizes))
@@ -126,8 +132,10 @@ def test_small_batch(R, RT, G):
         b.close()
 
 
-def test_replays_and_parameter_changes():
-    b = Batch(40, 64, 1200, seed=2)
+@pytest.mark.parametrize("mode", MODES)
+def test_replays_and_parameter_changes(mode):
+    print("batch replays, mode", mode, flush=True)
+    b = Batch(40, 64, 1200, seed=2, mode=mode)
     try:
         b.params(64, 8, 96)
         for _ in range(4):
@@ -143,10 +151,11 @@ def test_replays_and_parameter_changes():
         b.close()
 
 
-def test_full_batch_512():
+@pytest.mark.parametrize("mode", MODES)
+def test_full_batch_512(mode):
     """512 instances, N log-uniform in [64, 4096], seed 0: the measured configuration (about 4 GB of A)."""
-    print("batch full 512", flush=True)
-    b = Batch(512, 64, 4096, seed=0)
+    print("batch full 512, mode", mode, flush=True)
+    b = Batch(512, 64, 4096, seed=0, mode=mode)
     try:
         b.params(64, 8, 1024)
         b.run()
@@ -164,8 +173,10 @@ def test_full_batch_512():
         b.close()
 
 
-def test_autotune_picks_a_candidate_and_caches(tmp_path):
-    b = Batch(32, 64, 900, seed=3)
+@pytest.mark.parametrize("mode", MODES)
+def test_autotune_picks_a_candidate_and_caches(tmp_path, mode):
+    print("batch autotune, mode", mode, flush=True)
+    b = Batch(32, 64, 900, seed=3, mode=mode)
     try:
         best = (C.c_uint32 * 3)()
         path = str(tmp_path / "autotune.txt")
@@ -187,5 +198,6 @@ def test_bad_parameters_are_errors():
         assert workloads.cs_batch_set_params(b.h, 16, 8, 64) == 1
         assert workloads.cs_batch_set_params(b.h, 64, 5, 64) == 1
         assert workloads.cs_batch_set_params(b.h, 64, 8, 0) == 1
+        assert workloads.cs_batch_set_mode(b.h, 2) == 1
     finally:
         b.close()
