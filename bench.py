@@ -190,6 +190,11 @@ def bench_batch(extra, fused_gbps, reps):
     ms = np.array(list(each), np.float64)
     st = (C.c_uint64 * 4)()
     check(W.cs_batch_stats(h, st))
+    phase_ms = []
+    for leaf in (2, 3, 4, 5):  # unit map, pass 1, column reductions, pass 2, each back to back
+        t = C.c_float()
+        check(W.cs_batch_time_stage(h, leaf, 10, C.byref(t)))
+        phase_ms.append(round(t.value, 4))
     check(W.cs_batch_destroy(h))
     n = sizes.astype(np.float64)
     nbytes = float(np.sum(2 * n * n * 4 + 13 * n * 4))
@@ -198,7 +203,9 @@ def bench_batch(extra, fused_gbps, reps):
     rec = {"instances": 512, "instances_per_s": round(512 / (mean * 1e-3), 1), "ms_mean": round(mean, 4),
            "ms_p50": round(float(np.percentile(ms, 50)), 4), "ms_p99": round(float(np.percentile(ms, 99)), 4),
            "algorithmic_bytes": int(nbytes), "GBps": round(rate, 1), "per_byte_rate_vs_fused_16384": round(rate / fused_gbps, 3),
-           "params_R_RT_G": list(best), "autotune_s": round(tune_s, 2), "replays": reps, "executor_stats": list(st)}
+           "params_R_RT_G": list(best), "autotune_s": round(tune_s, 2), "replays": reps, "executor_stats": list(st),
+           "form": "phased: plan, device scan, unit map, pass 1, column reductions, pass 2 (six launches, one replayed graph)",
+           "phase_ms_back_to_back": phase_ms}
     extra["batched_ragged"] = rec
     log(f"batched: {rec}")
 
@@ -256,8 +263,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 1 GiB copy that scales the HBM counters, then exit")
-    ap.add_argument("--batch", action="store_true",
-                    help="also time the batched ragged GEMVER (off by default until its persistent kernel is verified on a GPU)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched ragged GEMVER")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (how the path would shard is "
@@ -306,10 +312,8 @@ def main():
                           "GBps_fused_bytes": round(gbs(fused_bytes, pms), 1), "fused_speedup_vs_plain": round(pms / ms, 3)}
         log(f"plain (4 components, B materialised): {pms * 1e3:.1f} us/step; fused speedup {pms / ms:.2f}x")
         bench_ns(extra, max(20, a.steps // 4))
-        if a.batch:
-            bench_batch(extra, value, max(30, a.steps * 3))
-        else:
-            extra["batched_ragged"] = "not measured: the persistent kernel's fix is not yet verified on a GPU (--batch runs it)"
+        if not a.no_batch:
+            bench_batch(extra, value, max(30, a.steps // 2))
         bench_precision(extra, max(20, a.steps // 8))
 
     cpu = None

@@ -91,6 +91,7 @@ WORKLOADS_API = [
     ("cs_batch_set_params", _I, [_P, _U32, _U32, _U32]),
     ("cs_batch_get_params", _I, [_P, C.POINTER(_U32)]),
     ("cs_batch_set_mode", _I, [_P, C.c_int]),
+    ("cs_batch_time_stage", _I, [_P, _I, _I, C.POINTER(_F)]),
     ("cs_batch_autotune", _I, [_P, _I, C.c_char_p, C.POINTER(_U32)]),
     ("cs_batch_run", _I, [_P]),
     ("cs_batch_wait", _I, [_P]),

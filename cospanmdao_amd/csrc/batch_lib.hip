@@ -1,8 +1,8 @@
 // cospanmdao_amd/csrc/batch_lib.hip — the batched ragged GEMVER model behind the C boundary, with its autotuner.
 //
 // A handle owns a cs::Model whose root is workloads/batch_systems.hpp's BatchedGemver, in its phased form (plan, scan,
-// pass 1, column reductions, pass 2: five launches) unless cs_batch_set_mode asks for the persistent one (plan, scan, one
-// work-queue kernel). Runs after the first two are replays of one captured graph. The autotuner times every candidate
+// unit map, pass 1, column reductions, pass 2: six launches) unless cs_batch_set_mode asks for the persistent one (plan,
+// scan, one work-queue kernel). Runs after the first two are replays of one captured graph. The autotuner times every candidate
 // (strip rows R, pass-2 rows RT and, persistent form only, work-groups G) on the model itself and keeps the fastest;
 // results are cached per device, problem and form in a small text file so later processes skip the search.
 
@@ -309,6 +309,13 @@ int cs_batch_autotune(cs_batch g, int reps, const char* cache_path, uint32_t* be
         }
         if (best3)
             for (int i = 0; i < 3; ++i) best3[i] = g->params[i];
+    });
+}
+
+int cs_batch_time_stage(cs_batch g, int leaf, int iters, float* ms) {
+    return wl::guard([&] {
+        if (!g || !ms || leaf < 0 || iters <= 0) throw std::invalid_argument("cs_batch_time_stage: bad argument");
+        *ms = model(*g).time_leaf(static_cast<std::size_t>(leaf), iters);
     });
 }
 

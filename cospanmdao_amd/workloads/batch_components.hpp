@@ -12,27 +12,34 @@
 //                    0 V  = N_k                                   vector elements          (x, w, u1 .. z offsets)
 //                    1 A4 = N_k * ld_k / 4                        16-byte units of A_k     (A offsets)
 //                    2 P  = strips_k * N_k                        strip partials           (P offsets)
-//                    3 U1 = strips_k * blocks_k                   pass-1 work units
-//                    4 UR = blocks_k                              column-reduction units
-//                    5 U2 = ceil(N_k / RT)                        pass-2 work units
-//                  (strips_k = ceil(N_k / R), blocks_k = ceil(N_k / 1024))
+//                    3 U1 = strips_k * ceil(N_k / 4096)           pass-1 tiles (R rows x 4096 columns)
+//                    4 UR = ceil(N_k / 1024)                      column-reduction blocks
+//                    5 U2 = ceil(N_k / RT)                        pass-2 row groups
+//                  (strips_k = ceil(N_k / R))
 //   ColumnScan     (custom op) ONE u32 exclusive scan of the 6K counts (cs_hip_build_column: decoupled look-back, wrap
 //                  flag): column c's offsets are scan[cK + k] - scan[cK]; column 0 is the int32 scan of the sizes.
-//   RaggedGemver   (persistent) a grid of G work-groups fetches work units by ticket (start order) until none is left:
-//                  first every pass-1 tile of every instance, then every column reduction, then every pass-2 row group. A
-//                  reduction of instance k waits until all of k's pass-1 tiles are done, a pass-2 unit until all of k's
-//                  reductions are: every wait is on units with earlier tickets, held by work-groups that are running them
-//                  and never wait on later ones, so the queue drains without co-residency. The last work-group to leave
-//                  zeroes the counters (graph replays need no memset).
-//   RaggedPass1 / RaggedReduce / RaggedPass2 (the phased form) the same units as three plain launches, one work-group per
-//                  unit (work-group u of a phase runs that phase's unit u): no work queue and no waits between
-//                  work-groups; the launch boundaries order the phases.
-// The arithmetic of a tile is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
+// then one of two forms:
+//   phased (default)
+//   UnitMap        (K work-groups) writes, for every unit of every phase, what the unit needs to start (instance size,
+//                  unit index within the instance, the instance's vector / A / P offsets) at the unit's scan position:
+//                  a phase's work-group then finds its unit with one load instead of a search over the scan.
+//   RaggedPass1 / RaggedReduce / RaggedPass2  one plain launch per phase, work-group u runs that phase's unit u: no
+//                  work queue and no waits between work-groups; the launch boundaries order the phases.
+//   persistent
+//   RaggedGemver   a grid of G work-groups fetches work units by ticket (start order) until none is left: first every
+//                  pass-1 tile of every instance, then every column reduction, then every pass-2 row group. A reduction of
+//                  instance k waits until all of k's pass-1 tiles are done, a pass-2 unit until all of k's reductions
+//                  are: every wait is on units with earlier tickets, held by work-groups that are running them and never
+//                  wait on later ones, so the queue drains without co-residency. The last work-group to leave zeroes the
+//                  counters (graph replays need no memset).
+// The arithmetic of a unit is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
 // the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4 or 8) and G are run-time parameters
 // (the autotuner's knobs; G only for the persistent form).
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
+#include <string>
 
 #include "naga/cospan/cospan.hpp"
 
@@ -42,8 +49,10 @@ namespace cs = naga::cospan;
 using cs::DType;
 
 inline constexpr int kWg    = 256;
-inline constexpr int kCols  = kWg * 4;  // columns of one pass-1 tile / one reduction unit / one pass-2 chunk
+inline constexpr int kCols  = kWg * 4;    // columns of one column-reduction block / one pass-2 step (4 per work-item)
+inline constexpr int kTile1 = kCols * 4;  // columns of one pass-1 tile: 4 interleaved chunks of kCols (16 per work-item)
 inline constexpr int kMaxRT = 8;
+inline constexpr int kMapWords = 8;       // u32 words per unit-map entry
 
 __host__ __device__ inline std::uint32_t ceil_div(std::uint32_t a, std::uint32_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline std::uint32_t ld_of(std::uint32_t n) { return (n + 3u) & ~3u; }
@@ -59,18 +68,25 @@ struct Knobs {
 inline void count_units(const std::uint32_t* sizes, std::size_t K, std::uint32_t R, std::uint32_t RT, std::uint64_t (&u)[3]) {
     u[0] = u[1] = u[2] = 0;
     for (std::size_t k = 0; k < K; ++k) {
-        const std::uint32_t n = sizes[k], bl = ceil_div(n, kCols);
-        u[0] += static_cast<std::uint64_t>(ceil_div(n, R)) * bl;
-        u[1] += bl;
+        const std::uint32_t n = sizes[k];
+        u[0] += static_cast<std::uint64_t>(ceil_div(n, R)) * ceil_div(n, kTile1);
+        u[1] += ceil_div(n, kCols);
         u[2] += ceil_div(n, RT);
     }
 }
+
+/// What a unit needs to start: its instance's size and offsets (vector elements, 16-byte units of A, P elements) and its
+/// index among the instance's units of its phase.
+struct Unit {
+    std::uint32_t n, t, vo, a4, poff;
+};
 
 /// The device scan's six offset columns (column c of instance k = scan[cK + k] - scan[cK]).
 struct Cols {
     const std::uint32_t* s;
     std::uint32_t        K;
     __device__ std::uint32_t off(int c, std::uint32_t k) const { return s[c * K + k] - s[c * K]; }
+    __device__ std::uint32_t count(int c, std::uint32_t k) const { return s[c * K + k + 1] - s[c * K + k]; }
     __device__ std::uint32_t total(int c) const { return s[(c + 1) * K] - s[c * K]; }
     /// the instance whose range of column c holds unit u: the largest k with off(c, k) <= u
     __device__ std::uint32_t find(int c, std::uint32_t u) const {
@@ -84,77 +100,89 @@ struct Cols {
         }
         return lo;
     }
+    __device__ Unit unit(const std::uint32_t* sizes, int c, std::uint32_t k, std::uint32_t u) const {
+        return Unit{sizes[k], u - off(c, k), off(0, k), off(1, k), off(2, k)};
+    }
 };
 
 /// What the units read and write (raw device pointers; a phase leaves the arrays it does not touch null).
 struct Arrays {
-    const std::uint32_t* sizes = nullptr;
-    Cols                 cols{nullptr, 0};
-    std::uint32_t        R     = 0;
-    const float *        A = nullptr, *u1 = nullptr, *v1 = nullptr, *u2 = nullptr, *v2 = nullptr, *y = nullptr, *z = nullptr;
-    const float*         x_in  = nullptr;  ///< x as pass 2 reads it
-    const float*         P_in  = nullptr;  ///< P as the column reduction reads it
-    float *              P = nullptr, *x = nullptr, *w = nullptr;
-    float                alpha = 0.0f, beta = 0.0f;
+    std::uint32_t R    = 0;
+    const float * A = nullptr, *u1 = nullptr, *v1 = nullptr, *u2 = nullptr, *v2 = nullptr, *y = nullptr, *z = nullptr;
+    const float*  x_in = nullptr;  ///< x as pass 2 reads it
+    const float*  P_in = nullptr;  ///< P as the column reduction reads it
+    float *       P = nullptr, *x = nullptr, *w = nullptr;
+    float         alpha = 0.0f, beta = 0.0f;
 };
 
 // ---- the units ------------------------------------------------------------------------------------------------------
 
-/// pass-1 tile `t` of instance k: strip c = t / blocks, column block b = t % blocks.
-__device__ inline void unit_pass1(const Arrays& a, std::uint32_t k, std::uint32_t t, std::uint32_t tid) {
-    const std::uint32_t n = a.sizes[k], ld = ld_of(n), blocks = ceil_div(n, kCols), R = a.R;
-    const std::uint32_t c = t / blocks, b = t % blocks;
-    const std::uint32_t j0 = b * kCols + tid * 4;
-    if (j0 >= n) return;
-    const std::uint32_t i0 = c * R, i1 = i0 + R < n ? i0 + R : n;
-    const std::size_t   vo = a.cols.off(0, k);
-    const float*        A  = a.A + 4 * static_cast<std::size_t>(a.cols.off(1, k));
-    const float *       u1 = a.u1 + vo, *v1 = a.v1 + vo, *u2 = a.u2 + vo, *v2 = a.v2 + vo, *y = a.y + vo;
-    float*              P  = a.P + a.cols.off(2, k);
-    float               acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (j0 + 4 <= n) {
-        using F4      = float __attribute__((ext_vector_type(4)));
-        const F4 a1   = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
-        const F4 b1   = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
-        const float* ap = A + static_cast<std::size_t>(i0) * ld + j0;
-#pragma unroll 8
-        for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
-            const F4    r  = __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap));
-            const float yi = y[i], p1 = u1[i], p2 = u2[i];
+/// pass-1 tile t of an instance: strip c = t / blocks (R rows), column block b = t % blocks (4096 columns). Work-item
+/// `tid` owns the 4 chunks of 4 columns at b 4096 + q 1024 + 4 tid (q < 4), so that each load instruction of a wave reads
+/// one contiguous 1 KiB run; a chunk that starts inside the row is loaded whole (a row holds ld >= n columns) and only its
+/// columns below n are kept. P[c][j] = the fma chain over the strip's rows, ascending.
+__device__ inline void unit_pass1(const Arrays& a, const Unit& w, std::uint32_t tid) {
+    using F4                = float __attribute__((ext_vector_type(4)));
+    const std::uint32_t n   = w.n, ld = ld_of(n), blocks = ceil_div(n, kTile1), R = a.R;
+    const std::uint32_t c   = w.t / blocks, b = w.t % blocks;
+    const std::uint32_t jb  = b * kTile1 + tid * 4;
+    if (jb >= n) return;
+    const std::uint32_t nq  = (n - 1 - jb) / kCols + 1 < 4 ? (n - 1 - jb) / kCols + 1 : 4;  // chunks that start inside the row
+    const std::uint32_t i0  = c * R, i1 = i0 + R < n ? i0 + R : n;
+    const float*        A   = a.A + 4 * static_cast<std::size_t>(w.a4);
+    const float *       u1  = a.u1 + w.vo, *v1 = a.v1 + w.vo, *u2 = a.u2 + w.vo, *v2 = a.v2 + w.vo, *y = a.y + w.vo;
+    F4                  a1[4], b1[4], acc[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) acc[e] = fmaf(fmaf(p2, b1[e], fmaf(p1, a1[e], r[e])), yi, acc[e]);
+    for (std::uint32_t q = 0; q < 4; ++q) {
+        acc[q] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (std::uint32_t e = 0; e < 4; ++e) {
+            const std::uint32_t j = jb + q * kCols + e;
+            a1[q][e]              = j < n ? v1[j] : 0.0f;
+            b1[q][e]              = j < n ? v2[j] : 0.0f;
         }
-        for (int e = 0; e < 4; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
-    } else {
-        const std::uint32_t m = n - j0;
-        for (std::uint32_t i = i0; i < i1; ++i)
-            for (std::uint32_t e = 0; e < m; ++e)
-                acc[e] = fmaf(fmaf(u2[i], v2[j0 + e], fmaf(u1[i], v1[j0 + e], A[static_cast<std::size_t>(i) * ld + j0 + e])), y[i], acc[e]);
-        for (std::uint32_t e = 0; e < m; ++e) P[static_cast<std::size_t>(c) * n + j0 + e] = acc[e];
     }
+    const float* ap = A + static_cast<std::size_t>(i0) * ld + jb;
+#pragma unroll 2
+    for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
+        F4 r[4];
+#pragma unroll
+        for (std::uint32_t q = 0; q < 4; ++q)
+            r[q] = q < nq ? __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap + q * kCols)) : F4{0.0f, 0.0f, 0.0f, 0.0f};
+        const float yi = y[i], p1 = u1[i], p2 = u2[i];
+#pragma unroll
+        for (std::uint32_t q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[q][e] = fmaf(fmaf(p2, b1[q][e], fmaf(p1, a1[q][e], r[q][e])), yi, acc[q][e]);
+    }
+    float* P = a.P + w.poff + static_cast<std::size_t>(c) * n;
+#pragma unroll
+    for (std::uint32_t q = 0; q < 4; ++q)
+#pragma unroll
+        for (std::uint32_t e = 0; e < 4; ++e)
+            if (jb + q * kCols + e < n) P[jb + q * kCols + e] = acc[q][e];
 }
 
-/// reduction unit b of instance k: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
-__device__ inline void unit_reduce(const Arrays& a, std::uint32_t k, std::uint32_t b, std::uint32_t tid) {
-    const std::uint32_t n = a.sizes[k], strips = ceil_div(n, a.R);
-    const std::size_t   vo = a.cols.off(0, k);
-    const float*        P  = a.P_in + a.cols.off(2, k);
+/// column-reduction block b of an instance: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
+__device__ inline void unit_reduce(const Arrays& a, const Unit& w, std::uint32_t tid) {
+    const std::uint32_t n = w.n, b = w.t, strips = ceil_div(n, a.R);
+    const float*        P = a.P_in + w.poff;
     for (std::uint32_t j = b * kCols + tid; j < n && j < (b + 1) * kCols; j += kWg) {
         float acc = 0.0f;
 #pragma unroll 8
         for (std::uint32_t c = 0; c < strips; ++c) acc += P[static_cast<std::size_t>(c) * n + j];
-        a.x[vo + j] = fmaf(a.beta, acc, a.z[vo + j]);
+        a.x[w.vo + j] = fmaf(a.beta, acc, a.z[w.vo + j]);
     }
 }
 
-/// pass-2 unit g of instance k: rows [g RT, g RT + RT). Ends with a barrier (its LDS is reused by the next unit).
+/// pass-2 row group g = t of an instance: rows [g RT, g RT + RT). Ends with a barrier (its LDS is reused by the next
+/// unit).
 template <int RT>
-__device__ inline void unit_pass2(const Arrays& a, std::uint32_t k, std::uint32_t g, std::uint32_t tid) {
-    const std::uint32_t n = a.sizes[k], ld = ld_of(n);
-    const std::uint32_t r0 = g * RT, nr = n - r0 < static_cast<std::uint32_t>(RT) ? n - r0 : static_cast<std::uint32_t>(RT);
-    const std::size_t   vo = a.cols.off(0, k);
-    const float*        A  = a.A + 4 * static_cast<std::size_t>(a.cols.off(1, k));
-    const float *       u1 = a.u1 + vo, *v1 = a.v1 + vo, *u2 = a.u2 + vo, *v2 = a.v2 + vo, *x = a.x_in + vo;
+__device__ inline void unit_pass2(const Arrays& a, const Unit& w, std::uint32_t tid) {
+    const std::uint32_t n = w.n, ld = ld_of(n);
+    const std::uint32_t r0 = w.t * RT, nr = n - r0 < static_cast<std::uint32_t>(RT) ? n - r0 : static_cast<std::uint32_t>(RT);
+    const float*        A  = a.A + 4 * static_cast<std::size_t>(w.a4);
+    const float *       u1 = a.u1 + w.vo, *v1 = a.v1 + w.vo, *u2 = a.u2 + w.vo, *v2 = a.v2 + w.vo, *x = a.x_in + w.vo;
     float               acc[RT], p1[RT], p2[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
@@ -194,8 +222,19 @@ __device__ inline void unit_pass2(const Arrays& a, std::uint32_t k, std::uint32_
         if (cs::wave::lane_id() == 0) ws[wv][r] = v;
     }
     __syncthreads();
-    if (tid < nr) a.w[vo + r0 + tid] = a.alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
+    if (tid < nr) a.w[w.vo + r0 + tid] = a.alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
     __syncthreads();
+}
+
+/// A phase's unit u from the unit map (one 32-byte entry; `total` = the phase's unit count from the scan, `map_elems` =
+/// the map's size). False when there is no such unit, or when the scan's count does not fit the map (then UnitMap wrote
+/// nothing for the instances past its end and no entry is used).
+__device__ inline bool map_unit(const std::uint32_t* map, std::size_t map_elems, std::uint32_t u, std::uint32_t total, Unit& w) {
+    using U4    = std::uint32_t __attribute__((ext_vector_type(4)));
+    const U4 e0 = reinterpret_cast<const U4*>(map)[2 * static_cast<std::size_t>(u)];
+    const U4 e1 = reinterpret_cast<const U4*>(map)[2 * static_cast<std::size_t>(u) + 1];
+    w           = Unit{e0[0], e0[1], e0[2], e0[3], e1[0]};
+    return u < total && static_cast<std::size_t>(total) * kMapWords <= map_elems;
 }
 
 CS_COMPONENTS_BEGIN
@@ -214,12 +253,12 @@ public:
         const std::size_t   K  = in.sizes.size();
         const std::uint32_t n  = in.sizes[k];
         const std::uint32_t R  = in.params[std::size_t{0}], RT = in.params[std::size_t{1}];
-        const std::uint32_t st = ceil_div(n, R), bl = ceil_div(n, kCols);
+        const std::uint32_t st = ceil_div(n, R);
         out.counts[0 * K + k] = n;
         out.counts[1 * K + k] = n * ld_of(n) / 4u;
         out.counts[2 * K + k] = st * n;
-        out.counts[3 * K + k] = st * bl;
-        out.counts[4 * K + k] = bl;
+        out.counts[3 * K + k] = st * ceil_div(n, kTile1);
+        out.counts[4 * K + k] = ceil_div(n, kCols);
         out.counts[5 * K + k] = ceil_div(n, RT);
     }
 
@@ -284,9 +323,10 @@ public:
     };
 
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Arrays        a    = arrays(in, out);
-        const Cols&         cols = a.cols;
-        const std::uint32_t K    = cols.K;
+        const Arrays         a     = arrays(in, out);
+        const Cols           cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t* sizes = cs::raw::data(in.sizes);
+        const std::uint32_t  K     = cols.K;
         const std::uint32_t RT   = in.params[std::size_t{1}];
         const std::uint32_t T1 = cols.total(3), TR = cols.total(4), T2 = cols.total(5), T = T1 + TR + T2;
         std::uint32_t*      st = cs::raw::data(out.state);
@@ -307,21 +347,21 @@ public:
             if (u >= T) break;
             if (u < T1) {
                 const std::uint32_t k = cols.find(3, u);
-                unit_pass1(a, k, u - cols.off(3, k), tid);
+                unit_pass1(a, cols.unit(sizes, 3, k, u), tid);
                 complete(&p1_done[k], tid);
             } else if (u < T1 + TR) {
                 const std::uint32_t k = cols.find(4, u - T1);
                 wait_for(&p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
-                unit_reduce(a, k, u - T1 - cols.off(4, k), tid);
+                unit_reduce(a, cols.unit(sizes, 4, k, u - T1), tid);
                 complete(&red_done[k], tid);
             } else {
                 const std::uint32_t k = cols.find(5, u - T1 - TR);
                 wait_for(&red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
-                const std::uint32_t g = u - T1 - TR - cols.off(5, k);
+                const Unit w = cols.unit(sizes, 5, k, u - T1 - TR);
                 if (RT == 4)
-                    unit_pass2<4>(a, k, g, tid);
+                    unit_pass2<4>(a, w, tid);
                 else
-                    unit_pass2<8>(a, k, g, tid);
+                    unit_pass2<8>(a, w, tid);
             }
         }
         // leaving: the last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
@@ -356,8 +396,6 @@ public:
 
     __device__ static Arrays arrays(const Inputs& in, const Outputs& out) {
         Arrays a;
-        a.sizes = cs::raw::data(in.sizes);
-        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
         a.R     = in.params[std::size_t{0}];
         a.A     = cs::raw::data(in.A);
         a.u1    = cs::raw::data(in.u1);
@@ -409,25 +447,77 @@ private:
     std::shared_ptr<Knobs> knobs_;
 };
 
-// ---- the phased form: one plain launch per phase, work-group u runs unit u -----------------------------------------
+// ---- the phased form: the unit map, then one plain launch per phase, work-group u runs unit u ----------------------
+
+/// For every unit of the three phases, its Unit at its scan position (map1: pass-1 tiles, mapR: column-reduction blocks,
+/// map2: pass-2 row groups; kMapWords u32 per entry: n, t, vo, a4, poff, k, 0, 0). One work-group per instance.
+class UnitMap : public cs::Component<UnitMap> {
+public:
+    static constexpr int max_work_group_size = kWg;
+    struct Inputs {
+        cs::Input<DType::u32, 1> sizes, offs;
+    };
+    struct Outputs {
+        cs::Output<DType::u32, 1> map1, mapR, map2;
+    };
+    __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        using U4                 = std::uint32_t __attribute__((ext_vector_type(4)));
+        const Cols          cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t k    = static_cast<std::uint32_t>(it.get_group(0));
+        const std::uint32_t tid  = static_cast<std::uint32_t>(it.get_local_id(0));
+        if (k >= cols.K) return;
+        const std::uint32_t n = in.sizes[std::size_t{k}], vo = cols.off(0, k), a4 = cols.off(1, k), poff = cols.off(2, k);
+        std::uint32_t* const maps[3] = {cs::raw::data(out.map1), cs::raw::data(out.mapR), cs::raw::data(out.map2)};
+        const std::size_t    caps[3] = {out.map1.size() / kMapWords, out.mapR.size() / kMapWords, out.map2.size() / kMapWords};
+        for (int p = 0; p < 3; ++p) {
+            const std::uint32_t o = cols.off(3 + p, k), cnt = cols.count(3 + p, k);
+            if (static_cast<std::size_t>(o) + cnt > caps[p]) continue;  // a scan that disagrees with the plan: write nothing
+            U4* dst = reinterpret_cast<U4*>(maps[p]) + 2 * static_cast<std::size_t>(o);
+            for (std::uint32_t i = tid; i < cnt; i += kWg) {
+                dst[2 * i]     = U4{n, i, vo, a4};
+                dst[2 * i + 1] = U4{poff, k, 0u, 0u};
+            }
+        }
+    }
+
+    Inputs  inputs;
+    Outputs outputs;
+    void    declare() override {
+        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
+        declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_output(outputs.map1, "map1", cs::unspecified_shape<1>());
+        declare_output(outputs.mapR, "mapR", cs::unspecified_shape<1>());
+        declare_output(outputs.map2, "map2", cs::unspecified_shape<1>());
+    }
+    std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        return create_launch_config(cs::range_desc(cs::range<1>{inputs.sizes.shape()[0] * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+    }
+};
+
+/// A phase's grid (one work-group per unit) must stay inside its unit map: the map is sized for the smallest R and RT.
+inline std::size_t phase_items(std::uint64_t units, std::size_t map_elems, const char* who) {
+    if (units == 0 || units * kMapWords > map_elems)
+        throw std::logic_error(std::string{"batch: the "} + who + " grid of " + std::to_string(units) + " work-groups does not fit its unit map (" +
+                               std::to_string(map_elems / kMapWords) + " entries)");
+    return static_cast<std::size_t>(units) * kWg;
+}
 
 /// Pass 1 of every instance: P (strip partials) from A and u1 v1 u2 v2 y.
 class RaggedPass1 : public cs::Component<RaggedPass1> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
-        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::u32, 1> params, offs, map;
         cs::Input<DType::f32, 1> A, u1, v1, u2, v2, y;
     };
     struct Outputs {
         cs::Output<DType::f32, 1> P;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        Unit       w;
+        if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(3), w)) return;
         Arrays a;
-        a.sizes = cs::raw::data(in.sizes);
-        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
-        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
-        if (u >= a.cols.total(3)) return;  // a grid larger than the device plan's count: nothing to do
         a.R  = in.params[std::size_t{0}];
         a.A  = cs::raw::data(in.A);
         a.u1 = cs::raw::data(in.u1);
@@ -436,23 +526,23 @@ public:
         a.v2 = cs::raw::data(in.v2);
         a.y  = cs::raw::data(in.y);
         a.P  = cs::raw::data(out.P);
-        const std::uint32_t k = a.cols.find(3, u);
-        unit_pass1(a, k, u - a.cols.off(3, k), static_cast<std::uint32_t>(it.get_local_id(0)));
+        unit_pass1(a, w, static_cast<std::uint32_t>(it.get_local_id(0)));
     }
 
     Inputs  inputs;
     Outputs outputs;
     void    declare() override {
-        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.params, "params", 3);
         declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.map, "map", cs::unspecified_shape<1>());
         for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
                                {&inputs.y, "y"}})
             declare_input(*w, name, cs::unspecified_shape<1>());
         declare_output(outputs.P, "P", cs::unspecified_shape<1>());
     }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[0]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+        return create_launch_config(cs::range_desc(cs::range<1>{phase_items(knobs_->units[0], inputs.map.shape()[0], "pass 1")}),
+                                    cs::wg_desc(cs::range<1>{kWg}));
     }
     explicit RaggedPass1(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
 
@@ -465,33 +555,31 @@ class RaggedReduce : public cs::Component<RaggedReduce> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
-        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::u32, 1> params, offs, map;
         cs::Input<DType::f32, 1> P, z, beta;
     };
     struct Outputs {
         cs::Output<DType::f32, 1> x;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        Unit       w;
+        if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(4), w)) return;
         Arrays a;
-        a.sizes = cs::raw::data(in.sizes);
-        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
-        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
-        if (u >= a.cols.total(4)) return;
         a.R    = in.params[std::size_t{0}];
         a.P_in = cs::raw::data(in.P);
         a.z    = cs::raw::data(in.z);
         a.x    = cs::raw::data(out.x);
         a.beta = in.beta[std::size_t{0}];
-        const std::uint32_t k = a.cols.find(4, u);
-        unit_reduce(a, k, u - a.cols.off(4, k), static_cast<std::uint32_t>(it.get_local_id(0)));
+        unit_reduce(a, w, static_cast<std::uint32_t>(it.get_local_id(0)));
     }
 
     Inputs  inputs;
     Outputs outputs;
     void    declare() override {
-        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.params, "params", 3);
         declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.map, "map", cs::unspecified_shape<1>());
         declare_input(inputs.P, "P", cs::unspecified_shape<1>());
         declare_input(inputs.z, "z", cs::unspecified_shape<1>());
         declare_input(inputs.beta, "beta", 1);
@@ -499,7 +587,8 @@ public:
     }
     void prepare_forward() override { set_shape(outputs.x, shape_of(inputs.z)[0]); }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[1]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+        return create_launch_config(cs::range_desc(cs::range<1>{phase_items(knobs_->units[1], inputs.map.shape()[0], "column reduction")}),
+                                    cs::wg_desc(cs::range<1>{kWg}));
     }
     explicit RaggedReduce(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
 
@@ -512,18 +601,17 @@ class RaggedPass2 : public cs::Component<RaggedPass2> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
-        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::u32, 1> params, offs, map;
         cs::Input<DType::f32, 1> A, u1, v1, u2, v2, x, alpha;
     };
     struct Outputs {
         cs::Output<DType::f32, 1> w;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        Unit       w;
+        if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(5), w)) return;
         Arrays a;
-        a.sizes = cs::raw::data(in.sizes);
-        a.cols  = Cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
-        const std::uint32_t u = static_cast<std::uint32_t>(it.get_group(0));
-        if (u >= a.cols.total(5)) return;
         a.A     = cs::raw::data(in.A);
         a.u1    = cs::raw::data(in.u1);
         a.v1    = cs::raw::data(in.v1);
@@ -532,19 +620,19 @@ public:
         a.x_in  = cs::raw::data(in.x);
         a.w     = cs::raw::data(out.w);
         a.alpha = in.alpha[std::size_t{0}];
-        const std::uint32_t k = a.cols.find(5, u), g = u - a.cols.off(5, k), tid = static_cast<std::uint32_t>(it.get_local_id(0));
+        const std::uint32_t tid = static_cast<std::uint32_t>(it.get_local_id(0));
         if (in.params[std::size_t{1}] == 4)
-            unit_pass2<4>(a, k, g, tid);
+            unit_pass2<4>(a, w, tid);
         else
-            unit_pass2<8>(a, k, g, tid);
+            unit_pass2<8>(a, w, tid);
     }
 
     Inputs  inputs;
     Outputs outputs;
     void    declare() override {
-        declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.params, "params", 3);
         declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.map, "map", cs::unspecified_shape<1>());
         for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
                                {&inputs.x, "x"}})
             declare_input(*w, name, cs::unspecified_shape<1>());
@@ -553,7 +641,8 @@ public:
     }
     void prepare_forward() override { set_shape(outputs.w, shape_of(inputs.x)[0]); }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->units[2]) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
+        return create_launch_config(cs::range_desc(cs::range<1>{phase_items(knobs_->units[2], inputs.map.shape()[0], "pass 2")}),
+                                    cs::wg_desc(cs::range<1>{kWg}));
     }
     explicit RaggedPass2(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
 

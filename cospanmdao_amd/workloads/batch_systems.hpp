@@ -5,7 +5,7 @@
 // =============================================================================
 //
 // Leaves: plan (BatchPlan) -> scan (ColumnScan) -> gemver (RaggedGemver, persistent), or in the phased form
-// plan -> scan -> p1 (RaggedPass1) -> red (RaggedReduce) -> p2 (RaggedPass2). Ports (both forms): inputs sizes, params
+// plan -> scan -> map (UnitMap) -> p1 (RaggedPass1) -> red (RaggedReduce) -> p2 (RaggedPass2). Ports (both forms): inputs sizes, params
 // {R, RT, G}, A (all instances back to back, instance k at 16-byte unit offs[K + k] - offs[K]), u1 v1 u2 v2 y z (back to
 // back, instance k at offs[k]), alpha, beta; outputs offs (the device scan, 6K + 1 entries), result ([total, flag] of the
 // scan), x, w.
@@ -28,12 +28,14 @@ namespace batch {
 struct Layout {
     std::vector<std::uint32_t> sizes;
     std::uint64_t              vec_total = 0, a4_total = 0, p_max = 0;
+    std::uint64_t              units_max[3] = {0, 0, 0};  ///< the phases' unit counts at the smallest R and RT (4)
     explicit Layout(std::vector<std::uint32_t> s, std::uint32_t min_R) : sizes{std::move(s)} {
         for (std::uint32_t n : sizes) {
             vec_total += n;
             a4_total += static_cast<std::uint64_t>(n) * ld_of(n) / 4;
             p_max += static_cast<std::uint64_t>(ceil_div(n, min_R)) * n;
         }
+        count_units(sizes.data(), sizes.size(), min_R, 4, units_max);
     }
 };
 
@@ -62,13 +64,17 @@ public:
         add_subsystem(cs::subsystem_desc<ColumnScan>("scan"));
         connect("plan.counts", "scan.counts");
         if (phased_) {
+            add_subsystem(cs::subsystem_desc<UnitMap>("map"));
             add_subsystem(cs::subsystem_desc<RaggedPass1>("p1", knobs_));
             add_subsystem(cs::subsystem_desc<RaggedReduce>("red", knobs_));
             add_subsystem(cs::subsystem_desc<RaggedPass2>("p2", knobs_));
-            for (const char* leaf : {"p1.offs", "red.offs", "p2.offs"}) connect("scan.offs", leaf);
+            for (const char* leaf : {"map.offs", "p1.offs", "red.offs", "p2.offs"}) connect("scan.offs", leaf);
+            connect("map.map1", "p1.map");
+            connect("map.mapR", "red.map");
+            connect("map.map2", "p2.map");
             connect("p1.P", "red.P");
             connect("red.x", "p2.x");
-            tie_inputs<DType::u32, 1>({"plan.sizes", "p1.sizes", "red.sizes", "p2.sizes"}, "sizes");
+            tie_inputs<DType::u32, 1>({"plan.sizes", "map.sizes"}, "sizes");
             tie_inputs<DType::u32, 1>({"plan.params", "p1.params", "red.params", "p2.params"}, "params", cs::shape(3));
             for (const char* v : {"A", "u1", "v1", "u2", "v2"})
                 tie_inputs<DType::f32, 1>({std::string{"p1."} + v, std::string{"p2."} + v}, v);
@@ -79,6 +85,9 @@ public:
             promote<DType::f32, 1>("red.x", "x");
             promote<DType::f32, 1>("p2.w", "w");
             shape_hook("p1.P", cs::dim::fixed_at_run(layout_->p_max));
+            shape_hook("map.map1", cs::dim::fixed_at_run(kMapWords * layout_->units_max[0]));
+            shape_hook("map.mapR", cs::dim::fixed_at_run(kMapWords * layout_->units_max[1]));
+            shape_hook("map.map2", cs::dim::fixed_at_run(kMapWords * layout_->units_max[2]));
         } else {
             add_subsystem(cs::subsystem_desc<RaggedGemver>("gemver", knobs_));
             connect("scan.offs", "gemver.offs");

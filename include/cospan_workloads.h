@@ -88,8 +88,9 @@ int cs_ns_destroy(cs_ns m);
  * cospanmdao/include/naga/cospan/fused_component.hpp:207-213); the offsets come from the column build it uses for
  * ragged shapes (shape_hook.hpp:215-348). `count` instances with N_k = round(lo (hi/lo)^U_k) (U_k from the hash (seed, k));
  * each x_k = beta Â_k^T y_k + z_k, w_k = alpha Â_k x_k as GEMVER above. One forward = a plan kernel, ONE device scan of
- * the per-instance counts, then (phased form, the default) one launch per phase over every instance: pass 1, the column
- * reductions, pass 2 -- or (persistent form) ONE work-queue launch; runs from the second on replay one captured graph.
+ * the per-instance counts, then (phased form, the default) a unit map and one launch per phase over every instance:
+ * pass 1, the column reductions, pass 2 -- or (persistent form) ONE work-queue launch; runs from the second on replay one
+ * captured graph.
  * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (6 x count), "offs" (6 x count + 1: the scan),
  * "result" ([total, flag] of the scan), "x", "w", "u1" .. "z" (instances back to back).
  * params: R (strip rows, >= 32), RT (pass-2 rows: 4 or 8), G (persistent work-groups; the phased form ignores it).
@@ -112,6 +113,9 @@ int cs_batch_read(cs_batch b, const char* name, void* host, size_t bytes);
 int cs_batch_time(cs_batch b, int iters, float* ms_per_run);
 /* `iters` back-to-back runs with a device event after each: ms[i] = the i-th run's time (for percentiles) */
 int cs_batch_time_each(cs_batch b, int iters, float* ms);
+/* one leaf of the model (0 plan, 1 scan, then 2 unit map, 3 pass 1, 4 column reductions, 5 pass 2 in the phased form,
+ * or 2 the persistent kernel) run `iters` times back to back after the leaves before it: *ms = its average device time */
+int cs_batch_time_stage(cs_batch b, int leaf, int iters, float* ms);
 int cs_batch_stats(cs_batch b, uint64_t* stats4);
 int cs_batch_destroy(cs_batch b);
 

@@ -21,6 +21,7 @@
 namespace cs = naga::cospan;
 using GAPlan = cs::detail::GroupArgs<batch::BatchPlan>;
 using GAGem  = cs::detail::GroupArgs<batch::RaggedGemver>;
+using GAMap  = cs::detail::GroupArgs<batch::UnitMap>;
 using GAP1   = cs::detail::GroupArgs<batch::RaggedPass1>;
 using GARed  = cs::detail::GroupArgs<batch::RaggedReduce>;
 using GAP2   = cs::detail::GroupArgs<batch::RaggedPass2>;
@@ -56,7 +57,7 @@ static void emulate_plan(const void* bytes, const std::uint32_t grid[3], const s
         c[k]         = n;
         c[K + k]     = n * batch::ld_of(n) / 4;
         c[2 * K + k] = st * n;
-        c[3 * K + k] = st * bl;
+        c[3 * K + k] = st * ((n + batch::kTile1 - 1) / batch::kTile1);
         c[4 * K + k] = bl;
         c[5 * K + k] = (n + RT - 1) / RT;
     }
@@ -84,15 +85,15 @@ struct Emu {
         }
         return lo;
     }
-    /// pass-1 unit u (global over the instances); returns its instance
+    /// pass-1 unit u (global over the instances); returns its instance. A tile is R rows x kTile1 columns.
     std::uint32_t pass1(std::uint32_t u) const {
         const std::uint32_t k = find(3, u), t = u - off(3, k), n = sz[k], ld = batch::ld_of(n);
-        const std::uint32_t blocks = (n + batch::kCols - 1) / batch::kCols, c = t / blocks, b = t % blocks;
+        const std::uint32_t blocks = (n + batch::kTile1 - 1) / batch::kTile1, c = t / blocks, b = t % blocks;
         const std::size_t   vo = off(0, k);
         const float*        Ak = A + 4ull * off(1, k);
         float*              Pk = P + off(2, k);
         const std::uint32_t i0 = c * R, i1 = std::min(i0 + R, n);
-        for (std::uint32_t j = b * batch::kCols; j < std::min<std::uint32_t>(n, (b + 1) * batch::kCols); ++j) {
+        for (std::uint32_t j = b * batch::kTile1; j < std::min<std::uint32_t>(n, (b + 1) * batch::kTile1); ++j) {
             float acc = 0.0f;
             for (std::uint32_t i = i0; i < i1; ++i)
                 acc = std::fma(std::fma(U2[vo + i], V2[vo + j], std::fma(U1[vo + i], V1[vo + j], Ak[std::size_t(i) * ld + j])), Y[vo + i], acc);
@@ -126,18 +127,33 @@ struct Emu {
     }
 };
 
-/// The fields every ragged kernel reads: sizes, params, the scan (checked against its expected length).
+/// The fields every ragged kernel reads: params, the scan (checked against its expected length), sizes (the instances'
+/// sizes as the host knows them, for the kernels that read them from the unit map).
+static const std::uint32_t* g_sizes = nullptr;
+static std::uint32_t        g_K     = 0;
+
 template <class In>
 static Emu emu_common(const In& in, const char* who) {
     Emu e;
-    e.K  = static_cast<std::uint32_t>(in.sizes.size());
+    e.K  = g_K;
     e.s  = cs::raw::data(in.offs);
-    e.sz = cs::raw::data(in.sizes);
+    e.sz = g_sizes;
     e.R  = cs::raw::data(in.params)[0];
     e.RT = cs::raw::data(in.params)[1];
     CHECK(in.offs.size() == 6 * e.K + 1, "%s: offs holds %zu words, not %u", who, in.offs.size(), 6 * e.K + 1);
     CHECK(std::size_t(e.s[6 * e.K]) <= (std::size_t(1) << 31), "%s: the scan total %u is implausible", who, e.s[6 * e.K]);
     return e;
+}
+
+/// The unit map's entry for unit u of a phase must name the unit the scan puts there.
+static void check_map(const Emu& e, const std::uint32_t* map, std::size_t map_elems, int c, std::uint32_t u, const char* who) {
+    CHECK((std::size_t(u) + 1) * batch::kMapWords <= map_elems, "%s: unit %u is outside its map (%zu words)", who, u, map_elems);
+    if ((std::size_t(u) + 1) * batch::kMapWords > map_elems) return;
+    const std::uint32_t* m = map + std::size_t(u) * batch::kMapWords;
+    const std::uint32_t  k = e.find(c, u);
+    const bool ok = m[0] == e.sz[k] && m[1] == u - e.off(c, k) && m[2] == e.off(0, k) && m[3] == e.off(1, k) && m[4] == e.off(2, k) && m[5] == k;
+    CHECK(ok, "%s: map entry of unit %u reads n %u t %u vo %u a4 %u poff %u k %u; the scan says instance %u", who, u, m[0], m[1], m[2], m[3],
+          m[4], m[5], k);
 }
 
 // ---- RaggedGemver: the persistent work queue, its units in ticket order ----------------------------------------------
@@ -194,6 +210,32 @@ static void check_phase_grid(const GA& a, const std::uint32_t grid[3], const std
     CHECK(a.units[2] == want, "%s: %zu work-groups for %u units", who, (size_t)a.units[2], want);
 }
 
+static std::uint64_t g_map_launches = 0;
+
+static void emulate_map(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    ++g_map_launches;
+    const auto& a = static_cast<const GAMap*>(bytes)->args.head;
+    Emu         e;
+    e.K  = static_cast<std::uint32_t>(a.in.sizes.size());
+    e.s  = cs::raw::data(a.in.offs);
+    e.sz = cs::raw::data(a.in.sizes);
+    CHECK(e.K == g_K && a.in.offs.size() == 6 * e.K + 1, "map: %u instances, offs %zu", e.K, a.in.offs.size());
+    CHECK(grid[0] == e.K && block[0] == batch::kWg, "map: grid %u block %u for %u instances", grid[0], block[0], e.K);
+    std::uint32_t* maps[3] = {cs::raw::data(a.out.map1), cs::raw::data(a.out.mapR), cs::raw::data(a.out.map2)};
+    const std::size_t caps[3] = {a.out.map1.size() / batch::kMapWords, a.out.mapR.size() / batch::kMapWords, a.out.map2.size() / batch::kMapWords};
+    for (std::uint32_t k = 0; k < e.K; ++k)
+        for (int p = 0; p < 3; ++p) {
+            const std::uint32_t o = e.off(3 + p, k), cnt = e.s[(3 + p) * e.K + k + 1] - e.s[(3 + p) * e.K + k];
+            CHECK(std::size_t(o) + cnt <= caps[p], "map: phase %d of instance %u ends at %u, the map holds %zu", p, k, o + cnt, caps[p]);
+            if (std::size_t(o) + cnt > caps[p]) continue;
+            for (std::uint32_t i = 0; i < cnt; ++i) {
+                std::uint32_t* m = maps[p] + std::size_t(o + i) * batch::kMapWords;
+                const std::uint32_t v[8] = {e.sz[k], i, e.off(0, k), e.off(1, k), e.off(2, k), k, 0, 0};
+                std::memcpy(m, v, sizeof v);
+            }
+        }
+}
+
 static void emulate_pass1(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
     ++g_phase_launches[0];
     const auto& a = static_cast<const GAP1*>(bytes)->args.head;
@@ -203,7 +245,10 @@ static void emulate_pass1(const void* bytes, const std::uint32_t grid[3], const 
     check_phase_grid(a, grid, block, e.total(3), "pass 1");
     CHECK(a.out.P.size() >= e.total(2), "pass 1: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
     CHECK(a.in.A.size() >= 4ull * e.total(1), "pass 1: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
-    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(3)); ++u) e.pass1(u);
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(3)); ++u) {
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 3, u, "pass 1");
+        e.pass1(u);
+    }
 }
 
 static void emulate_reduce(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
@@ -213,7 +258,10 @@ static void emulate_reduce(const void* bytes, const std::uint32_t grid[3], const
     e.Pin = cs::raw::data(a.in.P), e.Z = cs::raw::data(a.in.z), e.X = cs::raw::data(a.out.x), e.beta = cs::raw::data(a.in.beta)[0];
     check_phase_grid(a, grid, block, e.total(4), "reduce");
     CHECK(a.out.x.size() >= e.total(0) && a.in.z.size() >= e.total(0), "reduce: vectors shorter than the scan's total");
-    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(4)); ++u) e.reduce(u);
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(4)); ++u) {
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 4, u, "reduce");
+        e.reduce(u);
+    }
 }
 
 static void emulate_pass2(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
@@ -224,7 +272,10 @@ static void emulate_pass2(const void* bytes, const std::uint32_t grid[3], const 
     e.V2 = cs::raw::data(a.in.v2), e.Xin = cs::raw::data(a.in.x), e.W = cs::raw::data(a.out.w), e.alpha = cs::raw::data(a.in.alpha)[0];
     check_phase_grid(a, grid, block, e.total(5), "pass 2");
     CHECK(a.out.w.size() >= e.total(0) && a.in.x.size() >= e.total(0), "pass 2: vectors shorter than the scan's total");
-    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(5)); ++u) e.pass2(u);
+    for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(5)); ++u) {
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 5, u, "pass 2");
+        e.pass2(u);
+    }
 }
 
 // ---- inputs: instance k's A from one flat hash stream, written on the host into the mock's device memory ------------
@@ -287,6 +338,10 @@ static void run_form(bool phased) {
     const std::uint32_t count = 24, seed = 1;
     std::vector<std::uint32_t> sizes(count);
     or_ragged_sizes(seed, count, 16, 700, sizes.data());
+    sizes[5] = 4096 + 1000;  // an instance wider than one pass-1 tile (two column blocks, the second a partial one)
+    sizes[9] = 4096;         // exactly one tile wide
+    g_sizes = sizes.data();
+    g_K     = count;
     const float alpha = 1.5f, beta = 1.2f;
     auto        layout = std::make_shared<batch::Layout>(sizes, batch::BatchedGemver::kMinR);
     auto        knobs  = std::make_shared<batch::Knobs>();
@@ -320,6 +375,7 @@ int main() {
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::index_kernel<batch::BatchPlan>), sizeof(GAPlan), emulate_plan);
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedGemver>), sizeof(GAGem),
                           emulate_gemver);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::UnitMap>), sizeof(GAMap), emulate_map);
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedPass1>), sizeof(GAP1), emulate_pass1);
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedReduce>), sizeof(GARed),
                           emulate_reduce);
@@ -332,6 +388,7 @@ int main() {
     CHECK(f.empty(), "mock: %s", f.c_str());
     CHECK(g_gemver_launches >= 4, "the persistent kernel ran %llu times", (unsigned long long)g_gemver_launches);
     for (int p = 0; p < 3; ++p) CHECK(g_phase_launches[p] >= 4, "phase %d ran %llu times", p, (unsigned long long)g_phase_launches[p]);
+    CHECK(g_map_launches >= 4, "the unit map ran %llu times", (unsigned long long)g_map_launches);
     if (g_failures) {
         std::printf("%d failure(s)\n", g_failures);
         return 1;

@@ -61,8 +61,9 @@ def counts_of(sizes, R, RT):
     n = sizes.astype(np.uint64)
     ld = (n + 3) // 4 * 4
     strips = (n + R - 1) // R
-    blocks = (n + 1023) // 1024
-    c = np.concatenate([n, n * ld // 4, strips * n, strips * blocks, blocks, (n + RT - 1) // RT])
+    tiles = (n + 4095) // 4096  # pass-1 tiles are R rows x 4096 columns
+    blocks = (n + 1023) // 1024  # column-reduction blocks
+    c = np.concatenate([n, n * ld // 4, strips * n, strips * tiles, blocks, (n + RT - 1) // RT])
     assert c.size == 6 * K
     return c.astype(np.uint32)
 
@@ -128,6 +129,23 @@ def test_small_batch(R, RT, G, mode):
         res = b.read("result", np.uint32, 2)
         assert res[0] == offs[-1] and res[1] == 0
         check_instances(b, range(24), R)
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_wide_instances(mode):
+    """Instances wider than one pass-1 tile (4096 columns): two and three column blocks, partial last blocks."""
+    print("batch wide, mode", mode, flush=True)
+    b = Batch(5, 3000, 9000, seed=5, mode=mode)
+    try:
+        assert int(b.sizes.max()) > 4096, b.sizes
+        b.params(64, 4, 128)
+        b.run()
+        counts = b.read("counts", np.uint32, 6 * 5)
+        assert np.array_equal(counts, counts_of(b.sizes, 64, 4))
+        worst = check_instances(b, range(5), 64)
+        print(f"sizes {b.sizes.tolist()}, worst rel-L2 {worst:.2e}", flush=True)
     finally:
         b.close()
 
