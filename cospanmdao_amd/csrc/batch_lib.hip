@@ -117,7 +117,8 @@ cs::Model& model(cs_batch_s& g) {
 
 void set_params(cs_batch_s& g, std::uint32_t R, std::uint32_t RT, std::uint32_t G) {
     if (R < batch::BatchedGemver::kMinR) throw std::invalid_argument("batch: R must be at least 32 (the partials are sized for it)");
-    if (RT != 4 && RT != 8) throw std::invalid_argument("batch: RT is 4 or 8");
+    if (g.persistent ? (RT != 4 && RT != 8) : (RT != 4 && RT != 8 && RT != 16 && RT != 32))
+        throw std::invalid_argument(g.persistent ? "batch: RT is 4 or 8 (persistent form)" : "batch: RT is 4, 8, 16 or 32");
     if (G < 1 || G > 65536) throw std::invalid_argument("batch: the persistent grid is 1 .. 65536 work-groups");
     g.params = {R, RT, G};
     g.knobs->groups = static_cast<int>(G);
@@ -207,6 +208,8 @@ int cs_batch_set_mode(cs_batch g, int persistent) {
             if (g->model) g->model->wait();
             g->model.reset();  // rebuilt in the other form at the next use
             g->persistent = persistent != 0;
+            if (g->persistent && g->params[1] > 8) g->params[1] = 8;  // the persistent form takes RT 4 or 8
+            batch::count_units(g->layout->sizes.data(), g->layout->sizes.size(), g->params[0], g->params[1], g->knobs->units);
         }
     });
 }
@@ -290,8 +293,10 @@ int cs_batch_autotune(cs_batch g, int reps, const char* cache_path, uint32_t* be
             float         best = 1e30f;
             std::uint32_t bp[3] = {64, 8, static_cast<std::uint32_t>(4 * cus)};
             const std::vector<std::uint32_t> gms = g->persistent ? std::vector<std::uint32_t>{2u, 4u, 8u} : std::vector<std::uint32_t>{4u};
-            for (std::uint32_t R : {32u, 64u, 128u})
-                for (std::uint32_t RT : {4u, 8u})
+            const std::vector<std::uint32_t> Rs  = g->persistent ? std::vector<std::uint32_t>{32u, 64u, 128u} : std::vector<std::uint32_t>{64u, 128u, 256u};
+            const std::vector<std::uint32_t> RTs = g->persistent ? std::vector<std::uint32_t>{4u, 8u} : std::vector<std::uint32_t>{8u, 16u, 32u};
+            for (std::uint32_t R : Rs)
+                for (std::uint32_t RT : RTs)
                     for (std::uint32_t gm : gms) {
                         set_params(*g, R, RT, gm * static_cast<std::uint32_t>(cus));
                         m.run();  // eager (new parameters), then a capture, then replays

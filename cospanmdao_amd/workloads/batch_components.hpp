@@ -33,8 +33,8 @@
 //                  wait on later ones, so the queue drains without co-residency. The last work-group to leave zeroes the
 //                  counters (graph replays need no memset).
 // The arithmetic of a unit is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
-// the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4 or 8) and G are run-time parameters
-// (the autotuner's knobs; G only for the persistent form).
+// the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4 or 8; the phased form also 16 or
+// 32) and G are run-time parameters (the autotuner's knobs; G only for the persistent form).
 
 #include <cstdint>
 #include <memory>
@@ -50,7 +50,8 @@ using cs::DType;
 
 inline constexpr int kWg    = 256;
 inline constexpr int kCols  = kWg * 4;    // columns of one column-reduction block / one pass-2 step (4 per work-item)
-inline constexpr int kTile1 = kCols * 4;  // columns of one pass-1 tile: 4 interleaved chunks of kCols (16 per work-item)
+inline constexpr int kQ1    = 4;          // chunks of 4 columns per work-item in a pass-1 tile
+inline constexpr int kTile1 = kCols * kQ1;  // columns of one pass-1 tile: kQ1 interleaved chunks of kCols
 inline constexpr int kMaxRT = 8;
 inline constexpr int kMapWords = 8;       // u32 words per unit-map entry
 
@@ -117,47 +118,78 @@ struct Arrays {
 
 // ---- the units ------------------------------------------------------------------------------------------------------
 
-/// pass-1 tile t of an instance: strip c = t / blocks (R rows), column block b = t % blocks (4096 columns). Work-item
-/// `tid` owns the 4 chunks of 4 columns at b 4096 + q 1024 + 4 tid (q < 4), so that each load instruction of a wave reads
-/// one contiguous 1 KiB run; a chunk that starts inside the row is loaded whole (a row holds ld >= n columns) and only its
-/// columns below n are kept. P[c][j] = the fma chain over the strip's rows, ascending.
-__device__ inline void unit_pass1(const Arrays& a, const Unit& w, std::uint32_t tid) {
-    using F4                = float __attribute__((ext_vector_type(4)));
-    const std::uint32_t n   = w.n, ld = ld_of(n), blocks = ceil_div(n, kTile1), R = a.R;
-    const std::uint32_t c   = w.t / blocks, b = w.t % blocks;
-    const std::uint32_t jb  = b * kTile1 + tid * 4;
-    if (jb >= n) return;
-    const std::uint32_t nq  = (n - 1 - jb) / kCols + 1 < 4 ? (n - 1 - jb) / kCols + 1 : 4;  // chunks that start inside the row
-    const std::uint32_t i0  = c * R, i1 = i0 + R < n ? i0 + R : n;
-    const float*        A   = a.A + 4 * static_cast<std::size_t>(w.a4);
-    const float *       u1  = a.u1 + w.vo, *v1 = a.v1 + w.vo, *u2 = a.u2 + w.vo, *v2 = a.v2 + w.vo, *y = a.y + w.vo;
-    F4                  a1[4], b1[4], acc[4];
+namespace detail {
+/// The row loop of a pass-1 tile of NQ chunks (NQ uniform over the work-group). Chunks 0 .. NQ-2 start inside the row for
+/// every work-item; the last one only where `last` holds (within one wave the predicate differs only at the row's end).
+template <int NQ>
+__device__ inline void pass1_rows(const float* ap, std::uint32_t ld, std::uint32_t i0, std::uint32_t i1, bool last, const float* y,
+                                  const float* u1, const float* u2, const float (&a1)[kQ1][4], const float (&b1)[kQ1][4],
+                                  float (&acc)[kQ1][4]) {
+    using F4 = float __attribute__((ext_vector_type(4)));
+#pragma unroll 4
+    for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
+        F4 r[NQ];
 #pragma unroll
-    for (std::uint32_t q = 0; q < 4; ++q) {
-        acc[q] = F4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int q = 0; q < NQ - 1; ++q) r[q] = __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap + q * kCols));
+        r[NQ - 1] = last ? __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap + (NQ - 1) * kCols)) : F4{0.0f, 0.0f, 0.0f, 0.0f};
+        const float yi = y[i], p1 = u1[i], p2 = u2[i];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[q][e] = fmaf(fmaf(p2, b1[q][e], fmaf(p1, a1[q][e], r[q][e])), yi, acc[q][e]);
+    }
+}
+
+/// The same value in every lane, held in a scalar register (the unit descriptor is one per work-group): pointers built
+/// from it are uniform, so per-row values load through the scalar cache.
+__device__ inline std::uint32_t uniform(std::uint32_t v) { return static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v))); }
+}  // namespace detail
+
+/// pass-1 tile t of an instance: strip c = t / blocks (R rows), column block b = t % blocks (kTile1 columns). Work-item
+/// `tid` owns up to kQ1 chunks of 4 columns at b kTile1 + q 1024 + 4 tid (q < kQ1), so that each load instruction of a
+/// wave reads one contiguous 1 KiB run; a chunk that starts inside the row is loaded whole (a row holds ld >= n columns) and
+/// only its columns below n are kept. P[c][j] = the fma chain over the strip's rows, ascending.
+__device__ inline void unit_pass1(const Arrays& a, const Unit& wu, std::uint32_t tid) {
+    const Unit          w{detail::uniform(wu.n), detail::uniform(wu.t), detail::uniform(wu.vo), detail::uniform(wu.a4), detail::uniform(wu.poff)};
+    const std::uint32_t n = w.n, ld = ld_of(n), blocks = ceil_div(n, kTile1), R = a.R;
+    const std::uint32_t c = w.t / blocks, b = w.t % blocks;
+    const std::uint32_t cols = n - b * kTile1 < static_cast<std::uint32_t>(kTile1) ? n - b * kTile1 : static_cast<std::uint32_t>(kTile1);
+    const std::uint32_t nq   = ceil_div(cols, kCols);  // chunks of the tile (uniform)
+    const std::uint32_t jb   = b * kTile1 + tid * 4;
+    if (jb >= n) return;  // past the row's end in every chunk
+    const std::uint32_t i0 = c * R, i1 = i0 + R < n ? i0 + R : n;
+    const float*        A  = a.A + 4 * static_cast<std::size_t>(w.a4);
+    const float *       u1 = a.u1 + w.vo, *v1 = a.v1 + w.vo, *u2 = a.u2 + w.vo, *v2 = a.v2 + w.vo, *y = a.y + w.vo;
+    const bool          last = jb + (nq - 1) * kCols < n;  // the tile's last chunk starts inside the row
+    float               a1[kQ1][4], b1[kQ1][4], acc[kQ1][4];
+#pragma unroll
+    for (std::uint32_t q = 0; q < kQ1; ++q) {
 #pragma unroll
         for (std::uint32_t e = 0; e < 4; ++e) {
             const std::uint32_t j = jb + q * kCols + e;
             a1[q][e]              = j < n ? v1[j] : 0.0f;
             b1[q][e]              = j < n ? v2[j] : 0.0f;
+            acc[q][e]             = 0.0f;
         }
     }
     const float* ap = A + static_cast<std::size_t>(i0) * ld + jb;
-#pragma unroll 2
-    for (std::uint32_t i = i0; i < i1; ++i, ap += ld) {
-        F4 r[4];
-#pragma unroll
-        for (std::uint32_t q = 0; q < 4; ++q)
-            r[q] = q < nq ? __builtin_nontemporal_load(reinterpret_cast<const F4*>(ap + q * kCols)) : F4{0.0f, 0.0f, 0.0f, 0.0f};
-        const float yi = y[i], p1 = u1[i], p2 = u2[i];
-#pragma unroll
-        for (std::uint32_t q = 0; q < 4; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) acc[q][e] = fmaf(fmaf(p2, b1[q][e], fmaf(p1, a1[q][e], r[q][e])), yi, acc[q][e]);
+    static_assert(kQ1 == 2 || kQ1 == 4, "cospan batch: a pass-1 tile holds 2 or 4 chunks per work-item");
+    if constexpr (kQ1 == 4) {
+        switch (nq) {
+            case 4: detail::pass1_rows<4>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc); break;
+            case 3: detail::pass1_rows<3>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc); break;
+            case 2: detail::pass1_rows<2>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc); break;
+            default: detail::pass1_rows<1>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc); break;
+        }
+    } else {
+        if (nq == 2)
+            detail::pass1_rows<2>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc);
+        else
+            detail::pass1_rows<1>(ap, ld, i0, i1, last, y, u1, u2, a1, b1, acc);
     }
     float* P = a.P + w.poff + static_cast<std::size_t>(c) * n;
 #pragma unroll
-    for (std::uint32_t q = 0; q < 4; ++q)
+    for (std::uint32_t q = 0; q < kQ1; ++q)
 #pragma unroll
         for (std::uint32_t e = 0; e < 4; ++e)
             if (jb + q * kCols + e < n) P[jb + q * kCols + e] = acc[q][e];
@@ -224,6 +256,58 @@ __device__ inline void unit_pass2(const Arrays& a, const Unit& w, std::uint32_t 
     __syncthreads();
     if (tid < nr) a.w[w.vo + r0 + tid] = a.alpha * ((ws[0][tid] + ws[1][tid]) + (ws[2][tid] + ws[3][tid]));
     __syncthreads();
+}
+
+/// pass-2 row group g = t of an instance, RT = 4 RTW rows: wave v of the work-group owns rows [g RT + v RTW, + RTW) and
+/// sweeps every column of them (lane l takes the 4 columns at 4 l + 256 s), then reduces each row over its lanes alone: no
+/// LDS and no barrier, and a unit carries 4x the rows of unit_pass2<RTW>.
+template <int RTW>
+__device__ inline void unit_pass2_waves(const Arrays& a, const Unit& wu, std::uint32_t tid) {
+    const Unit w{detail::uniform(wu.n), detail::uniform(wu.t), detail::uniform(wu.vo), detail::uniform(wu.a4), detail::uniform(wu.poff)};
+    constexpr std::uint32_t kLanes = 64, kStep = kLanes * 4;
+    const std::uint32_t     n = w.n, ld = ld_of(n), lane = tid % kLanes, wv = tid / kLanes;
+    const std::uint32_t     r0 = w.t * (4 * RTW) + wv * RTW;
+    if (r0 >= n) return;
+    const std::uint32_t nr = n - r0 < static_cast<std::uint32_t>(RTW) ? n - r0 : static_cast<std::uint32_t>(RTW);
+    const float*        A  = a.A + 4 * static_cast<std::size_t>(w.a4);
+    const float *       u1 = a.u1 + w.vo, *v1 = a.v1 + w.vo, *u2 = a.u2 + w.vo, *v2 = a.v2 + w.vo, *x = a.x_in + w.vo;
+    float               acc[RTW], p1[RTW], p2[RTW];
+#pragma unroll
+    for (int r = 0; r < RTW; ++r) {
+        const std::uint32_t i = r0 + (static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u);
+        p1[r]                 = u1[i];
+        p2[r]                 = u2[i];
+        acc[r]                = 0.0f;
+    }
+    using F4         = float __attribute__((ext_vector_type(4)));
+    std::uint32_t j0 = lane * 4;
+#pragma unroll 2
+    for (; j0 + 4 <= n; j0 += kStep) {
+        const F4 av = F4{v1[j0], v1[j0 + 1], v1[j0 + 2], v1[j0 + 3]};
+        const F4 bv = F4{v2[j0], v2[j0 + 1], v2[j0 + 2], v2[j0 + 3]};
+        const F4 xv = F4{x[j0], x[j0 + 1], x[j0 + 2], x[j0 + 3]};
+        F4       t[RTW];
+#pragma unroll
+        for (int r = 0; r < RTW; ++r) {
+            const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+            t[r]                   = __builtin_nontemporal_load(reinterpret_cast<const F4*>(A + static_cast<std::size_t>(r0 + rr) * ld + j0));
+        }
+#pragma unroll
+        for (int r = 0; r < RTW; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[r] = fmaf(fmaf(p2[r], bv[e], fmaf(p1[r], av[e], t[r][e])), xv[e], acc[r]);
+    }
+    if (j0 < n)
+        for (int r = 0; r < RTW; ++r) {
+            const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+            for (std::uint32_t j = j0; j < n; ++j)
+                acc[r] = fmaf(fmaf(p2[r], v2[j], fmaf(p1[r], v1[j], A[static_cast<std::size_t>(r0 + rr) * ld + j])), x[j], acc[r]);
+        }
+#pragma unroll
+    for (int r = 0; r < RTW; ++r) {
+        const float v = cs::wave::reduce(acc[r], cs::plus<float>{});
+        if (lane == 0 && static_cast<std::uint32_t>(r) < nr) a.w[w.vo + r0 + r] = a.alpha * v;
+    }
 }
 
 /// A phase's unit u from the unit map (one 32-byte entry; `total` = the phase's unit count from the scan, `map_elems` =
@@ -596,7 +680,8 @@ private:
     std::shared_ptr<Knobs> knobs_;
 };
 
-/// Pass 2 of every instance: w = alpha Â x, RT rows per work-group.
+/// Pass 2 of every instance: w = alpha Â x, RT rows per work-group (4 or 8: the work-group sweeps the columns of its rows
+/// together; 16 or 32: each wave owns RT / 4 rows).
 class RaggedPass2 : public cs::Component<RaggedPass2> {
 public:
     static constexpr int max_work_group_size = kWg;
@@ -621,10 +706,12 @@ public:
         a.w     = cs::raw::data(out.w);
         a.alpha = in.alpha[std::size_t{0}];
         const std::uint32_t tid = static_cast<std::uint32_t>(it.get_local_id(0));
-        if (in.params[std::size_t{1}] == 4)
-            unit_pass2<4>(a, w, tid);
-        else
-            unit_pass2<8>(a, w, tid);
+        switch (in.params[std::size_t{1}]) {
+            case 4: unit_pass2<4>(a, w, tid); break;
+            case 8: unit_pass2<8>(a, w, tid); break;
+            case 16: unit_pass2_waves<4>(a, w, tid); break;
+            default: unit_pass2_waves<8>(a, w, tid); break;  // 32: the host accepts no other value
+        }
     }
 
     Inputs  inputs;

@@ -93,7 +93,8 @@ int cs_ns_destroy(cs_ns m);
  * captured graph.
  * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (6 x count), "offs" (6 x count + 1: the scan),
  * "result" ([total, flag] of the scan), "x", "w", "u1" .. "z" (instances back to back).
- * params: R (strip rows, >= 32), RT (pass-2 rows: 4 or 8), G (persistent work-groups; the phased form ignores it).
+ * params: R (strip rows, >= 32), RT (pass-2 rows per work-group: 4 or 8; the phased form also 16 or 32, a wave per RT/4
+ * rows), G (persistent work-groups; the phased form ignores it).
  */
 typedef struct cs_batch_s* cs_batch;
 
@@ -103,8 +104,8 @@ int cs_batch_set_params(cs_batch b, uint32_t R, uint32_t RT, uint32_t groups);
 int cs_batch_get_params(cs_batch b, uint32_t* params3);
 /* 0: the phased form (default), 1: the persistent work-queue form; a change rebuilds the model at its next use */
 int cs_batch_set_mode(cs_batch b, int persistent);
-/* times every candidate (R in {32, 64, 128}, RT in {4, 8}, and for the persistent form G in {2, 4, 8} x CUs) on this
- * model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
+/* times every candidate (phased: R in {64, 128, 256}, RT in {8, 16, 32}; persistent: R in {32, 64, 128}, RT in {4, 8},
+ * G in {2, 4, 8} x CUs) on this model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
  * read first and appended to */
 int cs_batch_autotune(cs_batch b, int reps, const char* cache_path, uint32_t* best3);
 int cs_batch_run(cs_batch b);

@@ -369,6 +369,14 @@ static void run_form(bool phased) {
     for (int i = 0; i < 3; ++i) m.run();
     std::snprintf(what, sizeof what, "%s: after a parameter change (R 64, RT 8, G 17)", form);
     check_outputs(m, sizes, 64, seed, alpha, beta, what);
+    if (phased) {  // wave-level pass 2 (RT 32) and long strips
+        params = {256, 32, 5};
+        batch::count_units(sizes.data(), count, 256, 32, knobs->units);
+        m.update_input("params", cs::init::from_host(params));
+        for (int i = 0; i < 3; ++i) m.run();
+        std::snprintf(what, sizeof what, "%s: R 256, RT 32", form);
+        check_outputs(m, sizes, 256, seed, alpha, beta, what);
+    }
 }
 
 int main() {

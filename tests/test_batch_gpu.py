@@ -112,8 +112,10 @@ MODES = [pytest.param(PHASED, id="phased"), pytest.param(PERSISTENT_MODE, id="pe
 
 
 @pytest.mark.parametrize("mode", MODES)
-@pytest.mark.parametrize("R,RT,G", [(32, 4, 3), (32, 8, 64), (64, 8, 17), (128, 4, 512)])
+@pytest.mark.parametrize("R,RT,G", [(32, 4, 3), (32, 8, 64), (64, 8, 17), (128, 4, 512), (64, 16, 5), (256, 32, 9)])
 def test_small_batch(R, RT, G, mode):
+    if RT > 8 and mode == PERSISTENT_MODE:
+        pytest.skip("the persistent form takes RT 4 or 8")
     print("batch small", R, RT, G, "mode", mode, flush=True)
     b = Batch(24, 16, 700, seed=1, mode=mode)
     try:
@@ -199,7 +201,10 @@ def test_autotune_picks_a_candidate_and_caches(tmp_path, mode):
         best = (C.c_uint32 * 3)()
         path = str(tmp_path / "autotune.txt")
         check(workloads.cs_batch_autotune(b.h, 2, path.encode(), best))
-        assert best[0] in (32, 64, 128) and best[1] in (4, 8) and best[2] > 0
+        if mode == PHASED:
+            assert best[0] in (64, 128, 256) and best[1] in (8, 16, 32) and best[2] > 0
+        else:
+            assert best[0] in (32, 64, 128) and best[1] in (4, 8) and best[2] > 0
         assert os.path.exists(path)
         again = (C.c_uint32 * 3)()
         check(workloads.cs_batch_autotune(b.h, 2, path.encode(), again))  # from the cache
@@ -217,5 +222,8 @@ def test_bad_parameters_are_errors():
         assert workloads.cs_batch_set_params(b.h, 64, 5, 64) == 1
         assert workloads.cs_batch_set_params(b.h, 64, 8, 0) == 1
         assert workloads.cs_batch_set_mode(b.h, 2) == 1
+        assert workloads.cs_batch_set_params(b.h, 64, 12, 64) == 1
+        check(workloads.cs_batch_set_mode(b.h, PERSISTENT_MODE))
+        assert workloads.cs_batch_set_params(b.h, 64, 16, 64) == 1  # wave-level pass 2: phased form only
     finally:
         b.close()
