@@ -202,7 +202,8 @@ def bench_batch(extra, fused_gbps, reps):
     rate = gbs(nbytes, mean)
     rec = {"instances": 512, "instances_per_s": round(512 / (mean * 1e-3), 1), "ms_mean": round(mean, 4),
            "ms_p50": round(float(np.percentile(ms, 50)), 4), "ms_p99": round(float(np.percentile(ms, 99)), 4),
-           "algorithmic_bytes": int(nbytes), "GBps": round(rate, 1), "per_byte_rate_vs_fused_16384": round(rate / fused_gbps, 3),
+           "algorithmic_bytes": int(nbytes), "GBps": round(rate, 1),
+           "per_byte_rate_vs_fused_16384": round(rate / fused_gbps, 3) if fused_gbps else None,
            "params_R_RT_G": list(best), "autotune_s": round(tune_s, 2), "replays": reps, "executor_stats": list(st),
            "form": "phased: plan, device scan, unit map, pass 1, column reductions, pass 2 (six launches, one replayed graph)",
            "phase_ms_back_to_back": phase_ms}
@@ -264,6 +265,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 1 GiB copy that scales the HBM counters, then exit")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ragged GEMVER")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched ragged GEMVER (512 instances, N in [64, 4096]); prints its record, not the bench line")
     a = ap.parse_args()
     if a.gpus != 1:
         sys.exit(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (how the path would shard is "
@@ -274,6 +277,11 @@ def main():
     from cospanmdao_amd import check, hip
 
     check(hip.cs_hip_init(0))
+    if a.batch_only:
+        extra = {}
+        bench_batch(extra, None, max(30, a.steps // 2))
+        print(json.dumps(extra["batched_ragged"]), flush=True)
+        return
     n, R = a.n, a.R
     style = TWOPASS if a.style == "twopass" else TWOPASS_1
 
