@@ -86,6 +86,14 @@ WORKLOADS_API = [
     ("cs_ns_time", _I, [_P, _I, C.POINTER(_F)]),
     ("cs_ns_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
     ("cs_ns_destroy", _I, [_P]),
+    ("cs_chain_create", _I, [C.POINTER(_P), _I, _S, _S, _U32]),
+    ("cs_chain_run", _I, [_P]),
+    ("cs_chain_wait", _I, [_P]),
+    ("cs_chain_read", _I, [_P, C.c_char_p, C.c_void_p, _S]),
+    ("cs_chain_time", _I, [_P, _I, C.POINTER(_F)]),
+    ("cs_chain_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
+    ("cs_chain_stats", _I, [_P, C.POINTER(C.c_uint64)]),
+    ("cs_chain_destroy", _I, [_P]),
     ("cs_batch_create", _I, [C.POINTER(_P), _U32, _U32, _U32, _F, _F, _U32]),
     ("cs_batch_sizes", _I, [_P, C.POINTER(_U32), _S]),
     ("cs_batch_set_params", _I, [_P, _U32, _U32, _U32]),

@@ -83,6 +83,42 @@ int cs_ns_time(cs_ns m, int iters, float* ms_per_run);
 int cs_ns_time_launches(cs_ns m, int iters, float* ms, int cap, int* count);
 int cs_ns_destroy(cs_ns m);
 
+/* ---- the GEMVER -> BiCGK -> GEMVER chain ------------------------------------------------------------------------------
+ * replaces: the Chain System of examples/gemver-chain/systems.hpp:226-346 in its six styles and the driver's actions
+ * (main.cpp:154-188). G1: B1 = A + u1 v1^T + u2 v2^T, x1 = beta B1^T ya + za, w1 = alpha B1 x1; BiCGK: q = B1 x1,
+ * s = B1^T w1; G2 on A = B1 with the same u1 .. v2, y = q, z = s: B2, x2, w2. alpha = beta = 1/sqrt(n) (the reference's
+ * chain, chain_data.hpp:13). Inputs as for GEMVER (base_seed 0: the reference's generator); G2's u1 .. v2 equal G1's.
+ * Styles (systems.hpp:33-53):
+ *   CS_CHAIN_UNFUSED   every component its own launch
+ *   CS_CHAIN_PARALLEL  as unfused, BiCGK's two products an ordinary System of independent children
+ *   CS_CHAIN_FUSED     GEMVER's rank-2 update + partial column sums one launch; BiCGK's two products one launch
+ *   CS_CHAIN_ONEPASS   fused GEMVERs, BiCGK from one pass over B1
+ *   CS_CHAIN_NAIVE     three fused groups with global barriers (G1, BiCGK, G2)
+ *   CS_CHAIN_CRAFTED   three fused groups over local-memory tiles of 16 rows (R is ignored): BiCGK and G2's rank-2 update
+ *                      share one read of B1
+ * n: a multiple of 256 and of R. Names for cs_chain_read (f32): "x1" "w1" "q" "s" "x2" "w2" (n), "B1" "B2" (n x n),
+ * and the inputs "A" "u1a" .. "za" "u1b" .. "v2b".
+ */
+#define CS_CHAIN_UNFUSED 0
+#define CS_CHAIN_PARALLEL 1
+#define CS_CHAIN_FUSED 2
+#define CS_CHAIN_ONEPASS 3
+#define CS_CHAIN_NAIVE 4
+#define CS_CHAIN_CRAFTED 5
+
+typedef struct cs_chain_s* cs_chain;
+
+int cs_chain_create(cs_chain* c, int style, size_t n, size_t R, uint32_t base_seed);
+int cs_chain_run(cs_chain c);
+int cs_chain_wait(cs_chain c);
+int cs_chain_read(cs_chain c, const char* name, void* host, size_t bytes);
+/* `iters` chains back to back between two device events, then one wait: *ms_per_run = the average */
+int cs_chain_time(cs_chain c, int iters, float* ms_per_run);
+/* per-launch device times of one eager run, averaged over `iters` runs (launch order) */
+int cs_chain_time_launches(cs_chain c, int iters, float* ms, int cap, int* count);
+int cs_chain_stats(cs_chain c, uint64_t* stats4);
+int cs_chain_destroy(cs_chain c);
+
 /* ---- batched ragged GEMVER -------------------------------------------------------------------------------------------
  * No reference counterpart as one launch (the reference refuses ragged ports inside fused groups,
  * cospanmdao/include/naga/cospan/fused_component.hpp:207-213); the offsets come from the column build it uses for

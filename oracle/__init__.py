@@ -46,6 +46,9 @@ def lib():
             "or_col_reduce_f32": (None, [_S, _S, _P, _P, C.c_float, _P]),
             "or_gemv_f32": (None, [_S, _P, _P, C.c_float, _P]),
             "or_gemver_twopass_f32": (None, [_S, _S, _P, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float, _P, _P, _P]),
+            "or_chain_f64": (None, [_S] + [_P] * 7 + [C.c_double, C.c_double] + [_P] * 7),
+            "or_bicgk_onepass_f32": (None, [_S, _S, _P, _P, _P, _P, _P]),
+            "or_row_reduce_f32": (None, [_S, _S, _P, _P]),
             "or_exclusive_scan_u32": (None, [_P, _P, _S]),
             "or_exclusive_scan_u64": (None, [_P, _P, _S]),
             "or_build_column": (C.c_uint32, [C.c_int, _P, _S, _S, C.c_int, _P, _P, _P]),
@@ -112,6 +115,61 @@ def gemver_f64(inp, alpha, beta, store=0, A=None, row_scale=None):
     lib().or_gemver_f64(n, store, ptr(A), n, ptr(row_scale) if row_scale is not None else None,
                         *[ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")], alpha, beta, ptr(x), ptr(w))
     return x, w
+
+
+CHAIN_OUT = ("x1", "w1", "q", "s", "x2", "w2")
+
+
+def chain_scale(n):
+    """alpha = beta = 1/sqrt(n) in f32, as the chain's Systems initialise them."""
+    return np.float32(1.0) / np.sqrt(np.float32(n))
+
+
+def chain_f64(inp, alpha, beta, want_B2=False):
+    """The chain in FP64 (or_chain_f64): {x1 w1 q s x2 w2 [B2]}."""
+    n = inp["u1"].shape[0]
+    out = {k: np.empty(n) for k in CHAIN_OUT}
+    B2 = np.empty((n, n)) if want_B2 else None
+    lib().or_chain_f64(n, ptr(inp["A"]), *[ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")], float(alpha), float(beta),
+                       *[ptr(out[k]) for k in CHAIN_OUT], ptr(B2) if want_B2 else None)
+    if want_B2:
+        out["B2"] = B2
+    return out
+
+
+def chain_f32(inp, onepass, R):
+    """The chain in f32 with the components' evaluation order: the unfused / parallel / fused styles (onepass False) or
+    the onepass / naive / crafted styles (onepass True; crafted at R = 16). {B1 x1 w1 q s B2 x2 w2}."""
+    L = lib()
+    n = inp["u1"].shape[0]
+    ab = float(chain_scale(n))
+    vec = [ptr(inp[k]) for k in ("u1", "v1", "u2", "v2")]
+
+    def gemver(A, y, z):
+        B = np.empty((n, n), np.float32)
+        L.or_rank2_update_f32(n, R, ptr(A), *vec, ptr(B))
+        P = np.empty((n // R, n), np.float32)
+        L.or_gemvt_partial_f32(n, R, ptr(B), ptr(y), ptr(P))
+        x = np.empty(n, np.float32)
+        L.or_col_reduce_f32(n, n // R, ptr(P), ptr(z), ab, ptr(x))
+        w = np.empty(n, np.float32)
+        L.or_gemv_f32(n, ptr(B), ptr(x), ab, ptr(w))
+        return B, x, w
+
+    B1, x1, w1 = gemver(inp["A"], inp["y"], inp["z"])
+    q = np.empty(n, np.float32)
+    sv = np.empty(n, np.float32)
+    Ps = np.empty((n // R, n), np.float32)
+    if onepass:
+        Pq = np.empty((n, n // 256), np.float32)
+        L.or_bicgk_onepass_f32(n, R, ptr(B1), ptr(x1), ptr(w1), ptr(Ps), ptr(Pq))
+        L.or_row_reduce_f32(n, n // 256, ptr(Pq), ptr(q))
+    else:
+        L.or_gemv_f32(n, ptr(B1), ptr(x1), 1.0, ptr(q))
+        L.or_gemvt_partial_f32(n, R, ptr(B1), ptr(w1), ptr(Ps))
+    L.or_col_reduce_f32(n, n // R, ptr(Ps), None, 1.0, ptr(sv))
+    B2, x2, w2 = gemver(B1, q, sv)
+    return {"B1": B1, "x1": x1, "w1": w1, "q": q, "s": sv, "B2": B2, "x2": x2, "w2": w2}
 
 
 def rel_l2(got, want):

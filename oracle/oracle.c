@@ -8,6 +8,10 @@
  *   hash / generators     against the reference's own chain_data.hpp, compiled from its sources (oracle/_ref)
  *   or_gemver_f64         against chain_data.hpp's reference_chain (x1, w1 at alpha = beta = 1/sqrt(n)) through
  *                         oracle/_ref; closed forms (u = v = 0 gives the textbook GEMV; y = 0 gives x = z)
+ *   or_chain_f64          against reference_chain bit for bit (x1 w1 q s x2 w2) through oracle/_ref; B2 against the
+ *                         closed form A0 + 2 (u1 v1^T + u2 v2^T)
+ *   or_bicgk_onepass_f32, against or_chain_f64's q and s (tolerance), a brute-force scan of the DPP order at n = 256, and
+ *   or_row_reduce_f32     plain sums on integer-valued data (exact in f32)
  *   f32 components        against or_gemver_f64 (tolerance) and brute force at tiny n; rank-2 update bitwise
  *                         against a plain fmaf loop
  *   scans / columns       std::exclusive_scan known answers of tests/test_device_scan.cpp:57-87 and
@@ -240,6 +244,124 @@ void or_gemver_f64(uint64_t n, int store, const void* A, uint64_t lda, const flo
         double       sum = 0.0;
         for (uint64_t j = 0; j < n; ++j) sum += (stored(store, A, row_scale, (uint64_t)i, j, lda) + (a1 * v1[j] + a2 * v2[j])) * x[j];
         w[i] = alpha * sum;
+    }
+}
+
+/* ---- the chain in FP64: examples/gemver-chain/common/chain_data.hpp:88-147 ------------------------------------------
+ * B_t(i, j) = A0(i, j) + t (u1_i v1_j + u2_i v2_j) with t = 1 for B1 and t = 2 for B2 (G2 adds the same update to B1),
+ * recomputed on the fly (chain_data.hpp:99-102). GEMVER (:104-117): acc_j = sum_i B(i, j) y_i (i ascending), x_j = z_j +
+ * beta acc_j, w_i = alpha sum_j B(i, j) x_j (j ascending). BiCGK (:124-133): q_i = sum_j B1(i, j) x1_j, s_j = sum_i B1(i, j)
+ * w1_i (i ascending). Threads split independent indices only, so every sum keeps the reference's order. */
+static double chain_b(const float* A0, const float* u1, const float* v1, const float* u2, const float* v2, uint64_t n, uint64_t i,
+                      uint64_t j, int t) {
+    return (double)A0[i * n + j] + (double)t * ((double)u1[i] * (double)v1[j] + (double)u2[i] * (double)v2[j]);
+}
+
+static void chain_gemver(uint64_t n, int t, const float* A0, const float* u1, const float* v1, const float* u2, const float* v2,
+                         const double* y, const double* z, double alpha, double beta, double* x, double* w) {
+    const long long JB = 256;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long jb = 0; jb < (long long)n; jb += JB) {
+        double         acc[256];
+        const uint64_t je = (uint64_t)jb + JB < n ? (uint64_t)jb + JB : n;
+        for (uint64_t j = (uint64_t)jb; j < je; ++j) acc[j - jb] = 0.0;
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint64_t j = (uint64_t)jb; j < je; ++j) acc[j - jb] += chain_b(A0, u1, v1, u2, v2, n, i, j, t) * y[i];
+        for (uint64_t j = (uint64_t)jb; j < je; ++j) x[j] = z[j] + beta * acc[j - jb];
+    }
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        double sum = 0.0;
+        for (uint64_t j = 0; j < n; ++j) sum += chain_b(A0, u1, v1, u2, v2, n, (uint64_t)i, j, t) * x[j];
+        w[i] = alpha * sum;
+    }
+}
+
+void or_chain_f64(uint64_t n, const float* A0, const float* u1, const float* v1, const float* u2, const float* v2, const float* y,
+                  const float* z, double alpha, double beta, double* x1, double* w1, double* q, double* s, double* x2, double* w2,
+                  double* B2) {
+    double* yz = (double*)malloc(2 * n * sizeof(double));
+    for (uint64_t i = 0; i < n; ++i) {
+        yz[i]     = y[i];
+        yz[n + i] = z[i];
+    }
+    chain_gemver(n, 1, A0, u1, v1, u2, v2, yz, yz + n, alpha, beta, x1, w1);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        double sum = 0.0;
+        for (uint64_t j = 0; j < n; ++j) sum += chain_b(A0, u1, v1, u2, v2, n, (uint64_t)i, j, 1) * x1[j];
+        q[i] = sum;
+    }
+    const long long JB = 256;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long jb = 0; jb < (long long)n; jb += JB) {
+        const uint64_t je = (uint64_t)jb + JB < n ? (uint64_t)jb + JB : n;
+        for (uint64_t j = (uint64_t)jb; j < je; ++j) s[j] = 0.0;
+        for (uint64_t i = 0; i < n; ++i)
+            for (uint64_t j = (uint64_t)jb; j < je; ++j) s[j] += chain_b(A0, u1, v1, u2, v2, n, i, j, 1) * w1[i];
+    }
+    chain_gemver(n, 2, A0, u1, v1, u2, v2, q, s, alpha, beta, x2, w2);
+    if (B2) {
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < (long long)n; ++i)
+            for (uint64_t j = 0; j < n; ++j) B2[(uint64_t)i * n + j] = chain_b(A0, u1, v1, u2, v2, n, (uint64_t)i, j, 2);
+    }
+    free(yz);
+}
+
+/* ---- BiCGK from one pass, f32, in this build's order (no reference order to follow: components.hpp:283-296 sums each
+ * row's block with an implementation-defined group reduction) ----------------------------------------------------------
+ * Ps[c][j] = fma chain of r_i B(i, j) over the strip's rows (ascending). Pq[i][b] for the 256-column block b: lane l
+ * (0 .. 63) takes columns 256 b + 4 l .. + 3, d_l = fma(B3 p3, fma(B2 p2, fma(B1 p1, B0 p0))), and the block's value is
+ * the total of a wave inclusive scan of d (dpp_scan_total64). */
+static float dpp_scan_total64(const float* d) {
+    /* Hillis-Steele inside rows of 16 lanes (shifts 1, 2, 4, 8; a lane with no source adds 0), then lane 15 of rows 0
+     * and 2 added to rows 1 and 3, then lane 31 added to rows 2 and 3; lanes a step leaves out add 0. */
+    float x[64], y[64];
+    memcpy(x, d, sizeof x);
+    for (int sh = 1; sh < 16; sh <<= 1) {
+        for (int l = 0; l < 64; ++l) y[l] = x[l] + ((l % 16) >= sh ? x[l - sh] : 0.0f);
+        memcpy(x, y, sizeof x);
+    }
+    for (int l = 0; l < 64; ++l) {
+        const int row = l / 16;
+        y[l]          = x[l] + ((row == 1 || row == 3) ? x[row * 16 - 1] : 0.0f);
+    }
+    memcpy(x, y, sizeof x);
+    for (int l = 0; l < 64; ++l) y[l] = x[l] + (l >= 32 ? x[31] : 0.0f);
+    return y[63];
+}
+
+void or_bicgk_onepass_f32(uint64_t n, uint64_t R, const float* B, const float* p, const float* r, float* Ps, float* Pq) {
+    const uint64_t strips = n / R, blocks = n / 256;
+#pragma omp parallel for schedule(static)
+    for (long long c = 0; c < (long long)strips; ++c) {
+        float* ps = Ps + (uint64_t)c * n;
+        for (uint64_t j = 0; j < n; ++j) ps[j] = 0.0f;
+        for (uint64_t k = 0; k < R; ++k) {
+            const uint64_t i  = (uint64_t)c * R + k;
+            const float*   b  = B + i * n;
+            const float    ri = r[i];
+            for (uint64_t j = 0; j < n; ++j) ps[j] = fmaf(ri, b[j], ps[j]);
+            for (uint64_t blk = 0; blk < blocks; ++blk) {
+                float d[64];
+                for (int l = 0; l < 64; ++l) {
+                    const uint64_t j = blk * 256 + 4 * (uint64_t)l;
+                    d[l]             = fmaf(b[j + 3], p[j + 3], fmaf(b[j + 2], p[j + 2], fmaf(b[j + 1], p[j + 1], b[j] * p[j])));
+                }
+                Pq[i * blocks + blk] = dpp_scan_total64(d);
+            }
+        }
+    }
+}
+
+/* components.hpp:326-340: q[i] = sum_b Pq[i][b], b ascending */
+void or_row_reduce_f32(uint64_t n, uint64_t blocks, const float* Pq, float* q) {
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < (long long)n; ++i) {
+        float acc = 0.0f;
+        for (uint64_t b = 0; b < blocks; ++b) acc += Pq[(uint64_t)i * blocks + b];
+        q[i] = acc;
     }
 }
 

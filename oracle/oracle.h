@@ -60,6 +60,14 @@ void or_rank2_update_f32(uint64_t n, uint64_t R, const float* A, const float* u1
 void or_gemvt_partial_f32(uint64_t n, uint64_t R, const float* B, const float* y, float* P);
 void or_col_reduce_f32(uint64_t n, uint64_t strips, const float* P, const float* z, float scale, float* x);
 void or_gemv_f32(uint64_t n, const float* B, const float* x, float scale, float* y);
+
+/* the GEMVER -> BiCGK -> GEMVER chain in FP64 (chain_data.hpp:88-147); B2 (n x n) may be NULL */
+void or_chain_f64(uint64_t n, const float* A0, const float* u1, const float* v1, const float* u2, const float* v2, const float* y,
+                  const float* z, double alpha, double beta, double* x1, double* w1, double* q, double* s, double* x2, double* w2,
+                  double* B2);
+/* BiCGK's one-pass partials in this build's order: Ps (n / R x n), Pq (n x n / 256); and q from Pq */
+void or_bicgk_onepass_f32(uint64_t n, uint64_t R, const float* B, const float* p, const float* r, float* Ps, float* Pq);
+void or_row_reduce_f32(uint64_t n, uint64_t blocks, const float* Pq, float* q);
 /* the fused two-pass form on the CPU (Â never stored): x bitwise equal to the components' x for the same R */
 void or_gemver_twopass_f32(uint64_t n, uint64_t R, const float* A, const float* u1, const float* v1, const float* u2,
                            const float* v2, const float* y, const float* z, float alpha, float beta, float* P, float* x,

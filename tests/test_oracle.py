@@ -364,3 +364,72 @@ def test_hash_with_base_is_the_flat_stream():
     part = np.empty((3, 40), np.float32)
     L.or_fill_hash_f32_base(11, O.ptr(part), 3, 40, 40, 2 * 40)
     assert np.array_equal(part, full[2:])
+
+
+# ---- the GEMVER -> BiCGK -> GEMVER chain --------------------------------------------------------------------------------
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (the reference checkout is absent)")
+@pytest.mark.parametrize("n", [256, 512])
+def test_chain_f64_matches_reference_chain(n):
+    """or_chain_f64 restates chain_data.hpp:88-147; the reference's own code (oracle/_ref) gives the same bits."""
+    inp = O.gemver_inputs(n)
+    ab = 1.0 / math.sqrt(n)
+    want = {k: np.empty(n) for k in O.CHAIN_OUT}
+    O.ref().ref_chain(n, O.ptr(inp["A"]), *[O.ptr(inp[k]) for k in ("u1", "v1", "u2", "v2", "y", "z")],
+                      *[O.ptr(want[k]) for k in O.CHAIN_OUT])
+    got = O.chain_f64(inp, ab, ab, want_B2=True)
+    for k in O.CHAIN_OUT:
+        assert np.array_equal(got[k], want[k]), k
+    # B2 = A0 + 2 (u1 v1^T + u2 v2^T), the closed form of G2's update of B1
+    u1, v1, u2, v2 = (inp[k].astype(np.float64) for k in ("u1", "v1", "u2", "v2"))
+    B2 = inp["A"].astype(np.float64) + 2.0 * (np.outer(u1, v1) + np.outer(u2, v2))
+    assert np.array_equal(got["B2"], B2)
+
+
+def test_chain_f64_closed_form_bicgk():
+    """q = B1 x1 and s = B1^T w1 against numpy's dense products in FP64 (tolerance: another summation order)."""
+    n = 256
+    inp = O.gemver_inputs(n, base_seed=3)
+    got = O.chain_f64(inp, 0.5, 0.25)
+    u1, v1, u2, v2 = (inp[k].astype(np.float64) for k in ("u1", "v1", "u2", "v2"))
+    B1 = inp["A"].astype(np.float64) + np.outer(u1, v1) + np.outer(u2, v2)
+    x1 = inp["z"] + 0.25 * (B1.T @ inp["y"].astype(np.float64))
+    w1 = 0.5 * (B1 @ x1)
+    assert np.allclose(got["x1"], x1, rtol=1e-12, atol=1e-12) and np.allclose(got["w1"], w1, rtol=1e-12, atol=1e-12)
+    assert np.allclose(got["q"], B1 @ got["x1"], rtol=1e-12, atol=1e-10)
+    assert np.allclose(got["s"], B1.T @ got["w1"], rtol=1e-12, atol=1e-10)
+    B2 = B1 + np.outer(u1, v1) + np.outer(u2, v2)
+    x2 = got["s"] + 0.25 * (B2.T @ got["q"])
+    assert np.allclose(got["x2"], x2, rtol=1e-12, atol=1e-9)
+    assert np.allclose(got["w2"], 0.5 * (B2 @ got["x2"]), rtol=1e-12, atol=1e-9)
+
+
+def test_bicgk_onepass_on_integers_is_exact():
+    """On small integers every sum is exact in f32, so the one-pass partials must equal the exact block sums: pins which
+    rows and columns feed each entry of Ps and Pq, whatever the order."""
+    n, R = 512, 64
+    rng = np.random.default_rng(7)
+    B = rng.integers(-8, 9, (n, n)).astype(np.float32)
+    p = rng.integers(-4, 5, n).astype(np.float32)
+    r = rng.integers(-4, 5, n).astype(np.float32)
+    Ps = np.empty((n // R, n), np.float32)
+    Pq = np.empty((n, n // 256), np.float32)
+    O.lib().or_bicgk_onepass_f32(n, R, O.ptr(B), O.ptr(p), O.ptr(r), O.ptr(Ps), O.ptr(Pq))
+    Bi, pi, ri = B.astype(np.int64), p.astype(np.int64), r.astype(np.int64)
+    assert np.array_equal(Ps.astype(np.int64), (Bi * ri[:, None]).reshape(n // R, R, n).sum(axis=1))
+    assert np.array_equal(Pq.astype(np.int64), (Bi * pi[None, :]).reshape(n, n // 256, 256).sum(axis=2))
+    q = np.empty(n, np.float32)
+    O.lib().or_row_reduce_f32(n, n // 256, O.ptr(Pq), O.ptr(q))
+    assert np.array_equal(q.astype(np.int64), Bi @ pi)
+
+
+@pytest.mark.parametrize("onepass,R", [(False, 64), (True, 64), (True, 16)])
+def test_chain_f32_against_fp64(onepass, R):
+    """The f32 restatements of every style's evaluation order stay within the reference's tolerance of the FP64 chain."""
+    n = 512
+    inp = O.gemver_inputs(n)
+    ab = O.chain_scale(n)
+    want = O.chain_f64(inp, float(ab), float(ab), want_B2=True)
+    got = O.chain_f32(inp, onepass, R)
+    for k in O.CHAIN_OUT:
+        assert O.rel_l2(got[k], want[k]) < 1e-5, k
+    assert O.rel_l2(got["B2"].ravel(), want["B2"].ravel()) < 1e-6
