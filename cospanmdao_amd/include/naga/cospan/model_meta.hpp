@@ -282,8 +282,9 @@ struct ModelMeta {
         instances.emplace(inst->path, inst);
     }
 
-    /// Every instance of `gone` now names `keep`; definitions merge (two specified extents must agree).
-    void merge(const std::shared_ptr<VarMeta>& keep, const std::shared_ptr<VarMeta>& gone, const std::string& who) {
+    /// Every instance of `gone` now names `keep`; definitions merge (two specified extents must agree). Both are taken by
+    /// value: callers pass an instance's own `var`, which the loop below rewrites.
+    void merge(const std::shared_ptr<VarMeta> keep, const std::shared_ptr<VarMeta> gone, const std::string& who) {
         if (keep == gone) return;
         if (!dtype_compatible(keep->dtype, gone->dtype) || keep->rank != gone->rank)
             throw std::invalid_argument("naga::cospan: " + who + ": '" + keep->name + "' (" + dtype_name(keep->dtype) + ", rank " +

@@ -47,6 +47,7 @@ struct Kernel {
 
 struct State {
     std::map<const void*, Kernel>      kernels;
+    mock::KernelFn                     fallback;
     std::map<std::uintptr_t, std::size_t> allocs;  // start -> bytes
     std::vector<std::unique_ptr<Stream>> streams;  // [0] is the null stream
     std::vector<std::unique_ptr<Event>>  events;
@@ -180,6 +181,7 @@ void column_counts(const Src* in, std::size_t k, std::uint32_t* ext, std::uint32
 
 namespace mock {
 void register_kernel(const void* stub, std::size_t arg_bytes, KernelFn fn) { S().kernels[stub] = Kernel{arg_bytes, std::move(fn)}; }
+void set_fallback_kernel(KernelFn fn) { S().fallback = std::move(fn); }
 bool is_device(const void* p) { return device_range(p, 1); }
 Stats stats() { return S().stats; }
 std::string failure() { return S().failure; }
@@ -340,6 +342,16 @@ int cs_hip_fill_u8_async(void* dst, uint8_t value, size_t count, cs_hip_stream s
 
 int cs_hip_launch(const void* kernel, const uint32_t grid[3], const uint32_t block[3], size_t, void** args, cs_hip_stream s) {
     auto it = S().kernels.find(kernel);
+    if (it == S().kernels.end() && S().fallback) {
+        S().stats.launches++;
+        mock::KernelFn fn = S().fallback;
+        const std::uint32_t g0 = grid[0], g1 = grid[1], g2 = grid[2], b0 = block[0], b1 = block[1], b2 = block[2];
+        enqueue(s, [fn, g0, g1, g2, b0, b1, b2] {
+            const std::uint32_t gg[3] = {g0, g1, g2}, bb[3] = {b0, b1, b2};
+            fn(nullptr, gg, bb);
+        });
+        return CS_HIP_OK;
+    }
     if (it == S().kernels.end()) {
         note_failure("launch of a kernel with no CPU emulation");
         return fail(CS_HIP_ERR_RUNTIME, "mock: no emulation for this kernel");

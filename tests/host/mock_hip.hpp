@@ -20,6 +20,10 @@ using KernelFn = std::function<void(const void* args, const std::uint32_t grid[3
 /// Registers the CPU emulation of the kernel whose host stub is `stub`; its one argument has `arg_bytes` bytes.
 void register_kernel(const void* stub, std::size_t arg_bytes, KernelFn fn);
 
+/// Launches of kernels with no registered emulation run `fn` instead (with null argument bytes) rather than failing:
+/// for tests of host logic only (wiring, launch counts), whose kernels' results are not looked at.
+void set_fallback_kernel(KernelFn fn);
+
 /// True when p lies inside a live mock device allocation.
 bool is_device(const void* p);
 

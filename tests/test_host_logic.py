@@ -25,8 +25,18 @@ def run(name):
 
 
 def test_batched_gemver_host_logic():
-    """Plan, device scan and persistent kernel wired through the System: shapes, pointers, zeroed work-queue state at every
-    start, x bit for bit and w against FP64, over eager runs, a capture, replays and a parameter change."""
+    """Plan, device scan, unit map and the kernels of both forms wired through the System: shapes, pointers, grids against
+    the unit counts, zeroed work-queue state at every start, x bit for bit and w against FP64, over eager runs, a capture,
+    replays and parameter changes."""
     build()
     out = run("batch_host")
     assert "capture + replays" in out
+
+
+def test_chain_host_logic():
+    """The GEMVER -> BiCGK -> GEMVER chain in its six styles: every input has a source (connections into promoted and tied
+    aliases reach every member), every shape resolves, and each style issues the launches of its plan, eager and replayed."""
+    build()
+    out = run("chain_host")
+    for style, launches in (("unfused", 11), ("parallel", 11), ("fused", 8), ("onepass", 9), ("naive", 8), ("crafted", 8)):
+        assert f"{style}: {launches} launches per chain" in out
