@@ -152,6 +152,41 @@ def bench_ns(extra, steps):
     extra["ns8192_bf16"]["speedup_vs_f32"] = round(extra["ns8192_f32"]["fused"]["ms"] / extra["ns8192_bf16"]["fused"]["ms"], 3)
 
 
+# The reference's published n = 16384 chain on an RTX 4090 (examples/gemver-chain/README.md, results tables): ms per chain.
+CHAIN_REFERENCE_4090_MS = {"unfused": 11.41, "parallel": 11.43, "fused": 9.150, "onepass": 8.655, "naive": 8.531, "crafted": 8.359}
+CHAIN_STYLES = ("unfused", "parallel", "fused", "onepass", "naive", "crafted")
+
+
+def bench_chain(extra, iters, n=16384, R=64):
+    """The GEMVER -> BiCGK -> GEMVER chain in the reference's six styles: ms per chain over `iters` back-to-back chains
+    (graph replays), the reference's nominal GB/s (8 n^2 x 4 B per chain) and the launches of one eager chain."""
+    from cospanmdao_amd import check, workloads as W
+
+    rec = {"n": n, "R": R, "nominal_bytes": 8 * n * n * 4,
+           "reference_rtx4090_ms": CHAIN_REFERENCE_4090_MS, "styles": {}}
+    for k, style in enumerate(CHAIN_STYLES):
+        h = C.c_void_p()
+        check(W.cs_chain_create(C.byref(h), k, n, R, 0))
+        for _ in range(3):
+            check(W.cs_chain_run(h))
+        check(W.cs_chain_wait(h))
+        t = C.c_float()
+        check(W.cs_chain_time(h, iters, C.byref(t)))
+        lm = (C.c_float * 16)()
+        cnt = C.c_int()
+        check(W.cs_chain_time_launches(h, 2, lm, 16, C.byref(cnt)))
+        check(W.cs_chain_destroy(h))
+        ms = t.value
+        rec["styles"][style] = {"ms": round(ms, 4), "GBps_nominal": round(gbs(8 * n * n * 4, ms), 1),
+                                "launch_ms": [round(lm[i], 4) for i in range(min(cnt.value, 16))],
+                                "speedup_vs_reference_rtx4090": round(CHAIN_REFERENCE_4090_MS[style] / ms, 2)}
+    base = rec["styles"]["unfused"]["ms"]
+    for st in rec["styles"].values():
+        st["speedup_vs_unfused"] = round(base / st["ms"], 3)
+    extra["gemver_chain"] = rec
+    log(f"gemver chain n={n}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GBps_nominal']:.0f} GB/s nominal)" for k, v in rec["styles"].items()))
+
+
 def bench_precision(extra, steps):
     n = 32768
     rec = {}
@@ -265,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 1 GiB copy that scales the HBM counters, then exit")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ragged GEMVER")
+    ap.add_argument("--chain-only", action="store_true",
+                    help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched ragged GEMVER (512 instances, N in [64, 4096]); prints its record, not the bench line")
     a = ap.parse_args()
@@ -277,6 +314,11 @@ def main():
     from cospanmdao_amd import check, hip
 
     check(hip.cs_hip_init(0))
+    if a.chain_only:
+        extra = {}
+        bench_chain(extra, max(10, a.steps // 10))
+        print(json.dumps(extra["gemver_chain"]), flush=True)
+        return
     if a.batch_only:
         extra = {}
         bench_batch(extra, None, max(30, a.steps // 2))
@@ -323,6 +365,7 @@ def main():
         if not a.no_batch:
             bench_batch(extra, value, max(30, a.steps // 2))
         bench_precision(extra, max(20, a.steps // 8))
+        bench_chain(extra, max(10, a.steps // 10))
 
     cpu = None
     if not (a.no_cpu or a.quick):

@@ -10,6 +10,7 @@
 // their bodies leave to the compiler fixed here (so the CPU oracle restates it bit for bit):
 //   Rank2Update   B[i][j] = fma(u2_i, v2_j, fma(u1_i, v1_j, A[i][j])), strips of R rows       (components.hpp:85-97)
 //   GemvTPartial  P[c][j] = sum_{k<R} B[cR+k][j] y[cR+k] as an fma chain, k ascending         (components.hpp:142-155)
+//                 (both: a work-item owns 4 adjacent columns of a strip, 16-byte accesses; the reference's owns one)
 //   ColReduce<Z>  x[j] = fma(scale, sum_c P[c][j], z[j]) (Z) or scale * sum (c ascending)      (components.hpp:192-204)
 //   Gemv          out[i] = scale * group_reduce(fma chain over j = l, l + wg, ...)            (components.hpp:237-250)
 //
@@ -60,15 +61,27 @@ public:
     struct Outputs {
         cs::Output<DType::f32, 2> B;
     };
+    /// Item t = (strip c, columns 4 q .. 4 q + 3) over (n / R) x (n / 4) items (16-byte loads and stores; n % 4 == 0).
+    /// GemvTPartial uses the same mapping, so in a fused segment a work-item reads back only its own writes.
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const std::uint32_t n = u32(in.A.shape()[1]), R = in.R[std::size_t{0}], total = (n / R) * n;
+        using F4            = float __attribute__((ext_vector_type(4)));
+        const std::uint32_t n = u32(in.A.shape()[1]), R = in.R[std::size_t{0}], quads = n / 4, total = (n / R) * quads;
         const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        const std::size_t   lda = in.A.stride()[0], ldb = out.B.stride()[0];
         for (std::uint32_t t = u32(it.get_global_id(0)); t < total; t += stride) {
-            const std::uint32_t c = t / n, j = t - c * n;
-            const float         v1j = in.v1[cs::id<1>{j}], v2j = in.v2[cs::id<1>{j}];
+            const std::uint32_t c = t / quads, j = 4 * (t - c * quads);
+            const F4            v1 = *reinterpret_cast<const F4*>(cs::raw::data(in.v1) + j);
+            const F4            v2 = *reinterpret_cast<const F4*>(cs::raw::data(in.v2) + j);
+            const float*        a  = cs::raw::data(in.A) + static_cast<std::size_t>(c) * R * lda + j;
+            float*              b  = cs::raw::data(out.B) + static_cast<std::size_t>(c) * R * ldb + j;
+            const float *       u1 = cs::raw::data(in.u1) + static_cast<std::size_t>(c) * R, *u2 = cs::raw::data(in.u2) + static_cast<std::size_t>(c) * R;
+#pragma unroll 4
             for (std::uint32_t k = 0; k < R; ++k) {
-                const std::uint32_t i = c * R + k;
-                out.B[cs::id<2>{i, j}] = rank2(in.A[cs::id<2>{i, j}], in.u1[cs::id<1>{i}], v1j, in.u2[cs::id<1>{i}], v2j);
+                const F4 av = *reinterpret_cast<const F4*>(a + k * lda);
+                F4       bv;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) bv[e] = rank2(av[e], u1[k], v1[e], u2[k], v2[e]);
+                *reinterpret_cast<F4*>(b + k * ldb) = bv;
             }
         }
     }
@@ -93,7 +106,7 @@ public:
     }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
         const std::size_t n    = inputs.A.shape()[1];
-        const std::size_t work = (n / static_cast<std::size_t>(R_)) * n;
+        const std::size_t work = (n / static_cast<std::size_t>(R_)) * (n / 4);
         return create_launch_config(cs::range_desc(cs::range<1>{round_up(work, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
     }
 
@@ -111,17 +124,24 @@ public:
     struct Outputs {
         cs::Output<DType::f32, 2> P;
     };
+    /// Rank2Update's item mapping (strip c, 4 columns); each column's sum is one fma chain, k ascending.
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const std::uint32_t n = u32(in.B.shape()[1]), R = u32(in.B.shape()[0] / out.P.shape()[0]), total = (n / R) * n;
+        using F4            = float __attribute__((ext_vector_type(4)));
+        const std::uint32_t n = u32(in.B.shape()[1]), R = u32(in.B.shape()[0] / out.P.shape()[0]), quads = n / 4, total = (n / R) * quads;
         const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
+        const std::size_t   ldb    = in.B.stride()[0];
         for (std::uint32_t t = u32(it.get_global_id(0)); t < total; t += stride) {
-            const std::uint32_t c = t / n, j = t - c * n;
-            float               acc = 0.0f;
+            const std::uint32_t c = t / quads, j = 4 * (t - c * quads);
+            const float*        b = cs::raw::data(in.B) + static_cast<std::size_t>(c) * R * ldb + j;
+            const float*        y = cs::raw::data(in.y) + static_cast<std::size_t>(c) * R;
+            F4                  acc = F4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
             for (std::uint32_t k = 0; k < R; ++k) {
-                const std::uint32_t i = c * R + k;
-                acc                   = fmaf(in.B[cs::id<2>{i, j}], in.y[cs::id<1>{i}], acc);
+                const F4 bv = *reinterpret_cast<const F4*>(b + k * ldb);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[e] = fmaf(bv[e], y[k], acc[e]);
             }
-            out.P[cs::id<2>{c, j}] = acc;
+            *reinterpret_cast<F4*>(cs::raw::data(out.P) + static_cast<std::size_t>(c) * out.P.stride()[0] + j) = acc;
         }
     }
 
@@ -134,7 +154,7 @@ public:
         declare_output(outputs.P, "P", cs::unspecified_shape<2>());  // hooked by the owning System: (n / R, n)
     }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        const std::size_t range = outputs.P.shape()[0] * outputs.P.shape()[1];
+        const std::size_t range = outputs.P.shape()[0] * (outputs.P.shape()[1] / 4);
         return create_launch_config(cs::range_desc(cs::range<1>{round_up(range, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
     }
 };
@@ -207,9 +227,14 @@ public:
         const float         scale = in.scale[std::size_t{0}];
         const std::uint32_t l = u32(it.get_local_id(0)), wg = u32(it.get_local_range(0));
         const std::uint32_t g = u32(it.get_group(0)), ng = u32(it.get_group_range(0));
+        const std::size_t ldb = in.B.stride()[0];
+        const float*      x   = cs::raw::data(in.x);
         for (std::uint32_t i = g; i < n; i += ng) {
-            float acc = 0.0f;
-            for (std::uint32_t j = l; j < n; j += wg) acc = fmaf(in.B[cs::id<2>{i, j}], in.x[cs::id<1>{j}], acc);
+            const float* row = cs::raw::data(in.B) + static_cast<std::size_t>(i) * ldb;
+            float        acc = 0.0f;
+            // the loads of 8 steps in flight; the sum stays one fma chain over j = l, l + wg, ... (the order is the spec)
+#pragma unroll 8
+            for (std::uint32_t j = l; j < n; j += wg) acc = fmaf(row[j], x[j], acc);
             const float sum = cs::reduce_over_group(it.get_group(), acc, cs::plus<float>());
             if (l == 0) out.y[cs::id<1>{i}] = scale * sum;
         }
