@@ -300,6 +300,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true", help="only the 1 GiB copy that scales the HBM counters, then exit")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ragged GEMVER")
+    ap.add_argument("--ns-only", action="store_true",
+                    help="only the NS residual lines (4096^2 and 8192^2 f32, 8192^2 bf16); prints their record, not the bench line")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
     ap.add_argument("--batch-only", action="store_true",
@@ -314,6 +316,11 @@ def main():
     from cospanmdao_amd import check, hip
 
     check(hip.cs_hip_init(0))
+    if a.ns_only:
+        extra = {}
+        bench_ns(extra, max(20, a.steps // 4))
+        print(json.dumps({k: extra[k] for k in ("ns4096_f32", "ns8192_f32", "ns8192_bf16")}), flush=True)
+        return
     if a.chain_only:
         extra = {}
         bench_chain(extra, max(10, a.steps // 10))

@@ -18,9 +18,9 @@
 // intermediates in global memory (components.hpp:636-852).
 //
 // NsTile<S>: a work-group owns TY rows x (256 * V) columns; each work-item owns V = 4 consecutive columns (one 16-byte
-// load per field and row for f32 fields, 8 bytes for bf16) and marches down its rows with a sliding window of three rows
-// in registers, so every field element
-// is read from HBM once (plus two halo rows per tile). The x neighbours of a work-item's V columns come from the adjacent
+// load per field and row for f32 fields, 8 bytes for bf16) and marches down its rows with a sliding window of rows in
+// registers (i - 1, i, i + 1, and i + 2 loading while row i is computed), so every field element is read from HBM once
+// (plus two halo rows per tile) and a row's loads have a whole iteration to arrive. The x neighbours of a work-item's V columns come from the adjacent
 // lanes by DPP wave shifts; only lanes 0 and 63 load their halo column from memory. Tiles are mapped to XCDs in
 // contiguous runs, so the halo rows of consecutive tiles meet in the same L2. S is the storage of u, v, p: f32 or bf16.
 
@@ -149,46 +149,59 @@ public:
         const bool right_lane = lane < 63 && j0 + kVec < nx && live;
         const std::uint32_t jl = j0 == 0 ? nx - 1 : j0 - 1, jr = j0 + kVec >= nx ? 0 : j0 + kVec;
 
-        auto row = [&](const typename F::T* f, std::uint32_t i, float (&a)[kVec]) {
-            // cached loads: the halo rows a tile shares with its neighbours are then served by the L2
-            F::widen(*reinterpret_cast<const typename F::Raw*>(f + static_cast<std::size_t>(i) * ld + j0), a);
+        // A row of the window: the work-item's kVec columns of u, v, p and, for the lanes whose x neighbour is not in the
+        // adjacent lane (the wave's edge lanes, the periodic wrap), that neighbour from memory.
+        struct Row {
+            float u[kVec], v[kVec], p[kVec];
+            float ul, ur, vl, vr, pl, pr;
         };
-        const std::uint32_t im = i0 == 0 ? ny - 1 : i0 - 1;
-        float um[kVec], uc[kVec], up[kVec], vm[kVec], vc[kVec], vp[kVec], pm[kVec], pc[kVec], pp[kVec];
-        row(U, im, um);
-        row(V, im, vm);
-        row(P, im, pm);
-        row(U, i0, uc);
-        row(V, i0, vc);
-        row(P, i0, pc);
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (std::uint32_t i = i0; i < i1; ++i) {
-            const std::uint32_t ip = i + 1 == ny ? 0 : i + 1;
-            row(U, ip, up);
-            row(V, ip, vp);
-            row(P, ip, pp);
-            // x neighbours of this work-item's columns
-            float ul = from_left_lane(uc[kVec - 1]), ur = from_right_lane(uc[0]);
-            float vl = from_left_lane(vc[kVec - 1]), vr = from_right_lane(vc[0]);
-            float pl = from_left_lane(pc[kVec - 1]), pr = from_right_lane(pc[0]);
-            const std::size_t rowoff = static_cast<std::size_t>(i) * ld;
+        auto load_row = [&](std::uint32_t i, Row& w) {
+            // cached loads: the halo rows a tile shares with its neighbours are then served by the L2
+            const std::size_t off = static_cast<std::size_t>(i) * ld;
+            F::widen(*reinterpret_cast<const typename F::Raw*>(U + off + j0), w.u);
+            F::widen(*reinterpret_cast<const typename F::Raw*>(V + off + j0), w.v);
+            F::widen(*reinterpret_cast<const typename F::Raw*>(P + off + j0), w.p);
             if (!left_lane) {
-                ul = F::one(U[rowoff + jl]);
-                vl = F::one(V[rowoff + jl]);
-                pl = F::one(P[rowoff + jl]);
+                w.ul = F::one(U[off + jl]);
+                w.vl = F::one(V[off + jl]);
+                w.pl = F::one(P[off + jl]);
             }
             if (!right_lane) {
-                ur = F::one(U[rowoff + jr]);
-                vr = F::one(V[rowoff + jr]);
-                pr = F::one(P[rowoff + jr]);
+                w.ur = F::one(U[off + jr]);
+                w.vr = F::one(V[off + jr]);
+                w.pr = F::one(P[off + jr]);
+            }
+        };
+        auto wrap = [&](std::uint32_t i) { return i >= ny ? i - ny : i; };
+        // the window: rows i - 1 (m), i (c), i + 1 (n) and, loaded one iteration ahead of its use, i + 2 (n2)
+        Row m{}, c{}, n{}, n2{};  // halo slots a lane never loads stay 0
+        load_row(i0 == 0 ? ny - 1 : i0 - 1, m);
+        load_row(i0, c);
+        load_row(wrap(i0 + 1), n);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (std::uint32_t i = i0; i < i1; ++i) {
+            if (i + 1 < i1) load_row(wrap(i + 2), n2);  // in flight while row i is computed
+            // x neighbours of this work-item's columns: from the adjacent lanes, else from memory (loaded with the row)
+            float ul = from_left_lane(c.u[kVec - 1]), ur = from_right_lane(c.u[0]);
+            float vl = from_left_lane(c.v[kVec - 1]), vr = from_right_lane(c.v[0]);
+            float pl = from_left_lane(c.p[kVec - 1]), pr = from_right_lane(c.p[0]);
+            if (!left_lane) {
+                ul = c.ul;
+                vl = c.vl;
+                pl = c.pl;
+            }
+            if (!right_lane) {
+                ur = c.ur;
+                vr = c.vr;
+                pr = c.pr;
             }
             float ru[kVec], rv[kVec], rc[kVec];
 #pragma unroll
             for (int e = 0; e < kVec; ++e) {
-                const float uw = e == 0 ? ul : uc[e - 1], ue = e == kVec - 1 ? ur : uc[e + 1];
-                const float vw = e == 0 ? vl : vc[e - 1], ve = e == kVec - 1 ? vr : vc[e + 1];
-                const float pw = e == 0 ? pl : pc[e - 1], pe = e == kVec - 1 ? pr : pc[e + 1];
-                cell(k, uc[e], ue, uw, up[e], um[e], vc[e], ve, vw, vp[e], vm[e], pe, pw, pp[e], pm[e], ru[e], rv[e], rc[e]);
+                const float uw = e == 0 ? ul : c.u[e - 1], ue = e == kVec - 1 ? ur : c.u[e + 1];
+                const float vw = e == 0 ? vl : c.v[e - 1], ve = e == kVec - 1 ? vr : c.v[e + 1];
+                const float pw = e == 0 ? pl : c.p[e - 1], pe = e == kVec - 1 ? pr : c.p[e + 1];
+                cell(k, c.u[e], ue, uw, n.u[e], m.u[e], c.v[e], ve, vw, n.v[e], m.v[e], pe, pw, n.p[e], m.p[e], ru[e], rv[e], rc[e]);
             }
             if (live) {
                 using F4 = float __attribute__((ext_vector_type(4)));
@@ -208,15 +221,9 @@ public:
                     s2 = fma(static_cast<double>(rc[e]), static_cast<double>(rc[e]), s2);
                 }
             }
-#pragma unroll
-            for (int e = 0; e < kVec; ++e) {
-                um[e] = uc[e];
-                uc[e] = up[e];
-                vm[e] = vc[e];
-                vc[e] = vp[e];
-                pm[e] = pc[e];
-                pc[e] = pp[e];
-            }
+            m = c;
+            c = n;
+            n = n2;
         }
         // the tile's partial: a balanced tree over the work-group (deterministic)
         const auto g = it.get_group();
