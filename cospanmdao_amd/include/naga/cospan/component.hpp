@@ -147,6 +147,83 @@ protected:
         ports_.back().inst->var->scratch = true;
     }
 
+    // ---- ragged ports (reference component.hpp:213-260): rank 2 only; a declaration gives nothing, N, or the counts ----
+    template <DType DT, int R>
+    void declare_input(RaggedInput<DT, R>& w, std::string name) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::input, DT, std::nullopt, nullptr, ragged_pointer(w));
+    }
+    template <DType DT, int R, std::integral N>
+    void declare_input(RaggedInput<DT, R>& w, std::string name, N n) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::input, DT, leading_of(n), nullptr, ragged_pointer(w));
+    }
+    template <DType DT, int R>
+    void declare_input(RaggedInput<DT, R>& w, std::string name, Counts c) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::input, DT, c.v.size(), &c.v, ragged_pointer(w));
+    }
+    template <DType DT, int R>
+    void declare_input(RaggedInput<DT, R>& w, std::string name, const Offsets& o) {
+        declare_input(w, std::move(name), counts_of(o));
+    }
+    template <DType DT, int R>
+    void declare_output(RaggedOutput<DT, R>& w, std::string name) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::output, DT, std::nullopt, nullptr, ragged_pointer(w));
+    }
+    template <DType DT, int R, std::integral N>
+    void declare_output(RaggedOutput<DT, R>& w, std::string name, N n) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::output, DT, leading_of(n), nullptr, ragged_pointer(w));
+    }
+    template <DType DT, int R>
+    void declare_output(RaggedOutput<DT, R>& w, std::string name, Counts c) {
+        add_ragged_port(&w, std::move(name), detail::IOKind::output, DT, c.v.size(), &c.v, ragged_pointer(w));
+    }
+    template <DType DT, int R>
+    void declare_output(RaggedOutput<DT, R>& w, std::string name, const Offsets& o) {
+        declare_output(w, std::move(name), counts_of(o));
+    }
+
+    /// The shape of a ragged port now: N and its counts (host or device); a dense variable reads as N rows of equal counts.
+    template <class W>
+        requires(W::ragged)
+    RaggedShape shape_of(const W& w) const {
+        const Port&            p = const_cast<ComponentBase*>(this)->port_of(&w, "shape_of");
+        const detail::VarMeta& v = *p.inst->var;
+        if (!v.shape_known())
+            throw std::invalid_argument("naga::cospan: shape_of: " + describe() + ": '" + v.name + "' has an unknown shape " + v.shape_text());
+        RaggedShape r;
+        r.n = v.dims[0];
+        if (v.per_index) {
+            r.total = v.total;
+            if (v.counts_on_host)
+                r.counts = v.counts;
+            else
+                r.column = v.column;
+        } else {
+            const std::size_t e = v.rank == 2 ? v.dims[1] : 1;
+            r.counts.assign(r.n, e);
+            r.total = static_cast<std::uint64_t>(r.n) * e;
+        }
+        return r;
+    }
+    /// A ragged output's N and counts from a RaggedShape (shape_of of a ragged input, or built).
+    template <DType DT, int R>
+    void set_shape(RaggedOutput<DT, R>& w, const RaggedShape& shape) {
+        detail::VarMeta& v = ragged_settable(&w);
+        if (shape.column)
+            v.set_device_counts(std::const_pointer_cast<detail::DeviceColumn>(shape.column));
+        else
+            v.set_host_counts(shape.counts, describe());
+    }
+    /// A ragged output's N (a number, or cs::as_is) and its host counts (counts.size() == N).
+    template <DType DT, int R, class N>
+    void set_shape(RaggedOutput<DT, R>& w, N n, std::vector<std::size_t> c) {
+        detail::VarMeta& v = ragged_settable(&w);
+        if constexpr (!std::is_same_v<N, AsIsTag>)
+            if (static_cast<std::size_t>(n) != c.size())
+                throw std::invalid_argument("naga::cospan: set_shape: " + describe() + ": N = " + std::to_string(static_cast<std::size_t>(n)) +
+                                            " but " + std::to_string(c.size()) + " counts");
+        v.set_host_counts(std::move(c), describe());
+    }
+
     /// The initial value of one of this component's inputs (the last declaration wins; ignored when the input is fed).
     void init_input(std::string_view name, InitSpec init) {
         Port&                     p = port_named(name, "init_input");
@@ -159,6 +236,7 @@ protected:
 
     /// Sets the run-time shape of an output (or an unspecified input dimension); cs::as_is leaves a dimension alone.
     template <class W, class... E>
+        requires(!requires { W::ragged; })
     void set_shape(W& w, E... dims) {
         static_assert(sizeof...(E) == W::rank, "cospan: set_shape takes one entry per dimension");
         Port&           p = port_of(&w, "set_shape");
@@ -177,6 +255,7 @@ protected:
 
     /// The run-time shape of a port (an error while it is unknown).
     template <class W>
+        requires(!requires { W::ragged; })
     range<W::rank> shape_of(const W& w) const {
         const Port&            p = const_cast<ComponentBase*>(this)->port_of(&w, "shape_of");
         const detail::VarMeta& v = *p.inst->var;
@@ -189,13 +268,65 @@ protected:
 
 private:
     friend struct detail::ComponentAccess;
+    using RaggedPoint = std::function<void(void*, std::size_t, const std::uint32_t*, const std::uint32_t*, std::size_t)>;
     struct Port {
         std::string                                                        name;
         detail::IOKind                                                     kind;
         const void*                                                        wrapper;
         std::function<void(void*, const std::size_t*, const std::size_t*)> point;
         std::shared_ptr<detail::Instance>                                  inst;
+        RaggedPoint                                                        point_ragged;  ///< set for a ragged port
     };
+
+    template <class W>
+    static RaggedPoint ragged_pointer(W& w) {
+        return [&w](void* d, std::size_t n, const std::uint32_t* o, const std::uint32_t* per, std::size_t ext) {
+            detail::WrapperAccess::point_ragged(w, d, n, o, per, ext);
+        };
+    }
+    template <std::integral N>
+    static std::size_t leading_of(N n) {
+        if constexpr (std::is_signed_v<N>)
+            if (n < 0) throw std::invalid_argument("naga::cospan: a ragged declaration: N cannot be negative");
+        return static_cast<std::size_t>(n);
+    }
+    static Counts counts_of(const Offsets& o) {
+        if (o.v.empty() || o.v[0] != 0) throw std::invalid_argument("naga::cospan: cs::offsets: the offsets must start at 0");
+        Counts c;
+        for (std::size_t i = 1; i < o.v.size(); ++i) {
+            if (o.v[i] < o.v[i - 1]) throw std::invalid_argument("naga::cospan: cs::offsets: the offsets decrease at " + std::to_string(i));
+            c.v.push_back(o.v[i] - o.v[i - 1]);
+        }
+        return c;
+    }
+    void add_ragged_port(const void* wrapper, std::string name, detail::IOKind kind, DType dt, std::optional<std::size_t> n,
+                         const std::vector<std::size_t>* c, RaggedPoint point) {
+        add_port(wrapper, std::move(name), kind, dt, 2, ShapeSpec{}, nullptr);
+        Port&            p = ports_.back();
+        detail::VarMeta& v = *p.inst->var;
+        p.point_ragged     = std::move(point);
+        p.inst->ragged     = true;
+        v.ragged_capable   = true;
+        if (n) {
+            v.dims[0]  = *n;
+            v.known[0] = v.fixed[0] = true;
+        }
+        if (c) {
+            v.set_host_counts(*c, describe());
+            v.counts_fixed = true;
+            v.fixed[0]     = true;
+        }
+    }
+    detail::VarMeta& ragged_settable(const void* w) {
+        Port&            p = port_of(w, "set_shape");
+        detail::VarMeta& v = *p.inst->var;
+        if (p.kind != detail::IOKind::output)
+            throw std::invalid_argument("naga::cospan: set_shape: " + describe() + "." + p.name + " is an input: inputs are never reshaped");
+        if (v.counts_fixed)
+            throw std::invalid_argument("naga::cospan: set_shape: " + describe() + "." + p.name + ": the counts of '" + v.name +
+                                        "' come from " + (v.hooked_ragged ? "a shape hook" : "its declaration") + "; set_shape cannot change them");
+        return v;
+    }
 
     void add_port(const void* wrapper, std::string name, detail::IOKind kind, DType dt, int rank, const ShapeSpec& shape,
                   std::function<void(void*, const std::size_t*, const std::size_t*)> point) {
@@ -251,6 +382,12 @@ public:
                 v.pending = false;  // an initializer is ignored once the input has a source
             } else if (v.pending) {
                 m.materialize(v, describe());
+            } else if (v.hooked_ragged) {
+                // a free ragged input whose shape a hook gives: it carries a shape; its data (zeroed) is never written
+                if (v.ensure_allocation()) detail::hip_check(cs_hip_fill_u8_async(v.data, 0, v.capacity, m.exec.stream(0)), "zeroing of '" + v.name + "'");
+                v.has_value = true;
+            } else if (!v.has_value && v.shape_known() && v.elements() == 0) {
+                v.has_value = true;  // an empty input needs no value
             } else if (!v.has_value && !v.data) {
                 throw std::invalid_argument("naga::cospan: " + describe() + ": input '" + p.name + "' (" + v.name +
                                             ") has no source and no initial value");
@@ -269,14 +406,32 @@ public:
                 detail::hip_check(cs_hip_fill_u8_async(v.data, 0, v.capacity, m.exec.stream(0)), "zeroing of '" + v.name + "'");
                 m.exec.after_init(v);
             }
-            std::size_t stride[3];
-            detail::row_major_strides(v, stride);
-            p.point(v.data, v.dims, stride);
+            if (p.point_ragged) {
+                if (v.per_index) {
+                    v.ensure_device_counts(m.exec.stream(0));
+                    p.point_ragged(v.data, v.dims[0], v.column->offs, v.column->extents, 0);
+                } else {
+                    p.point_ragged(v.data, v.dims[0], nullptr, nullptr, v.rank == 2 ? v.dims[1] : 1);  // a dense variable: rows
+                }
+            } else {
+                if (v.per_index)
+                    throw std::invalid_argument("naga::cospan: " + describe() + ": '" + v.name +
+                                                "' is ragged; a dense Input / Output cannot see per-index extents: declare '" + p.name +
+                                                "' with RaggedInput / RaggedOutput");
+                std::size_t stride[3];
+                detail::row_major_strides(v, stride);
+                p.point(v.data, v.dims, stride);
+            }
             (p.kind == detail::IOKind::input ? reads : writes).push_back(&v);
         }
     }
 
     virtual std::unique_ptr<LaunchDescription> describe_launch() = 0;
+
+    /// The variables of the ports and their kinds (where a ragged shape hook runs relative to the leaves).
+    void port_vars(std::vector<std::pair<detail::VarMeta*, detail::IOKind>>& out) const {
+        for (const Port& p : ports_) out.emplace_back(p.inst->var.get(), p.kind);
+    }
 };
 
 namespace detail {
@@ -414,6 +569,7 @@ public:
         args.head = kl->args;
         return launcher_.prepare(Split::pad, false, single_segment(), plans, args, m.exec.stream(0), path);
     }
+    void port_vars(std::vector<std::pair<VarMeta*, IOKind>>& out) const override { comp_->port_vars(out); }
 
 private:
     std::unique_ptr<X> comp_;
@@ -443,6 +599,7 @@ public:
         };
         return rec;
     }
+    void port_vars(std::vector<std::pair<VarMeta*, IOKind>>& out) const override { comp_->port_vars(out); }
 
 private:
     std::unique_ptr<X> comp_;

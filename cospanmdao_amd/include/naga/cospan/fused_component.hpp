@@ -129,6 +129,9 @@ public:
         // global barriers without group waits become launch boundaries (GroupLauncher::prepare)
         return launcher_.prepare(Sp, Waits, segs_, plans, args, m.exec.stream(0), path_);
     }
+    void port_vars(std::vector<std::pair<VarMeta*, IOKind>>& out) const override {
+        std::apply([&](const auto&... c) { ((c ? c->port_vars(out) : void()), ...); }, comps_);
+    }
 
 private:
     template <class X, class... A>

@@ -14,8 +14,16 @@
 // identical run replays it with one hipGraphLaunch. Any change (a resize, a new input shape, a different argument)
 // falls back to eager launches, and the next stable run captures again. update_input with the same shape reuses the
 // allocation, so value updates keep the graph.
+//
+// Ragged shape hooks (counts_from / offsets_from, reference shape_hook.hpp:400-473) need the VALUES of their source: such a
+// run goes leaf by leaf (prepare, launch), with each hook placed directly after the last writer of its source (else before
+// the first holder of its target); the hook waits for the device once, builds the target's column with one launch and
+// reads back [total, flags]. Those runs are eager: the column's total decides allocations, so no graph is captured.
 
+#include <algorithm>
+#include <cstdint>
 #include <memory>
+#include <span>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -65,6 +73,56 @@ private:
     std::vector<value_type>          data_;
 };
 
+/// A host copy of a rank-2 variable as N blocks end to end (reference host_view.hpp:155-196): a ragged variable's blocks
+/// have its per-index counts; a dense variable's blocks are its rows.
+template <DType DT, int Rank>
+class HostRagged {
+public:
+    static_assert(Rank == 2, "naga::cospan: a ragged host view is rank 2 only");
+    using value_type = cpp_type_t<DT>;
+    HostRagged(std::shared_ptr<detail::VarMeta> v, detail::Executor* ex) : var_{std::move(v)}, ex_{ex} {}
+
+    void sync() {
+        detail::VarMeta& v = *var_;
+        if (v.write) detail::hip_check(cs_hip_event_synchronize(v.write->get()), "view sync");
+        const cs_hip_stream s = ex_->stream(0);
+        n_                    = v.dims[0];
+        offs_.assign(n_ + 1, 0);
+        if (v.per_index && !v.counts_on_host) {
+            std::vector<std::uint32_t> o(n_ + 1);
+            detail::hip_check(cs_hip_memcpy_async(o.data(), v.column->offs, (n_ + 1) * 4, s), "view sync");
+            detail::hip_check(cs_hip_stream_synchronize(s), "view sync");
+            std::copy(o.begin(), o.end(), offs_.begin());
+        } else {
+            for (std::size_t i = 0; i < n_; ++i) offs_[i + 1] = offs_[i] + (v.per_index ? v.counts[i] : v.dims[1]);
+        }
+        data_.resize(v.elements());
+        if (!data_.empty()) {
+            detail::hip_check(cs_hip_memcpy_async(data_.data(), v.data, data_.size() * sizeof(value_type), s), "view sync");
+            detail::hip_check(cs_hip_stream_synchronize(s), "view sync");
+        }
+    }
+    std::size_t                       size() const { return data_.size(); }
+    const value_type*                 data() const { return data_.data(); }
+    std::span<const value_type>       span() const { return {data_.data(), data_.size()}; }
+    const value_type&                 operator[](std::size_t flat) const { return data_[flat]; }
+    /// N, the number of blocks.
+    std::size_t                       n() const { return n_; }
+    /// The N + 1 block offsets: block i is data()[offsets[i], offsets[i + 1]).
+    const std::vector<std::uint64_t>& block_offsets() const { return offs_; }
+    std::span<const value_type>       block(std::size_t i) const {
+        if (i + 1 >= offs_.size()) throw std::out_of_range("naga::cospan: HostRagged::block past the last block");
+        return {data_.data() + offs_[i], static_cast<std::size_t>(offs_[i + 1] - offs_[i])};
+    }
+
+private:
+    std::shared_ptr<detail::VarMeta> var_;
+    detail::Executor*                ex_;
+    std::size_t                      n_ = 0;
+    std::vector<std::uint64_t>       offs_;
+    std::vector<value_type>          data_;
+};
+
 /// Counters of a model's executor.
 struct ModelStats {
     std::uint64_t runs = 0, launches = 0, captures = 0, replays = 0;
@@ -110,6 +168,11 @@ public:
         detail::ModelMeta& m = *meta_;
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
+        if (const auto steps = ragged_steps(m); !steps.empty()) {
+            run_stepwise(m, steps);
+            ++stats_.runs;
+            return;
+        }
         std::vector<detail::LaunchRecord> recs;
         recs.reserve(m.leaves.size());
         std::vector<std::byte> sig;
@@ -155,12 +218,17 @@ public:
         detail::ModelMeta& m = *meta_;
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
+        const auto steps = ragged_steps(m);
         std::vector<detail::LaunchRecord> recs;
-        for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
+        if (steps.empty())
+            for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
         std::vector<detail::Executor::TimedLaunch> t;
         m.exec.set_timing(&t);
         try {
-            for (std::size_t k = 0; k < recs.size(); ++k) m.exec.submit(*m.leaves[k], recs[k]);
+            if (!steps.empty())
+                run_stepwise(m, steps);
+            else
+                for (std::size_t k = 0; k < recs.size(); ++k) m.exec.submit(*m.leaves[k], recs[k]);
         } catch (...) {
             m.exec.set_timing(nullptr);
             throw;
@@ -187,6 +255,7 @@ public:
         if (iters <= 0) throw std::invalid_argument("naga::cospan: time_leaf: iters must be positive");
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
+        if (!ragged_steps(m).empty()) throw std::invalid_argument("naga::cospan: time_leaf: a model with ragged shape hooks runs leaf by leaf; time it with run_timed");
         std::vector<detail::LaunchRecord> recs;
         for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
         if (k >= recs.size())
@@ -226,6 +295,16 @@ public:
         return view<DT, Rank>(path, detail::IOKind::input);
     }
 
+    /// A host view of a rank-2 variable as blocks (ragged: its per-index counts; dense: its rows).
+    template <DType DT, int Rank>
+    HostRagged<DT, Rank> view_ragged_output(std::string_view path) {
+        return view_ragged<DT, Rank>(path, detail::IOKind::output);
+    }
+    template <DType DT, int Rank>
+    HostRagged<DT, Rank> view_ragged_input(std::string_view path) {
+        return view_ragged<DT, Rank>(path, detail::IOKind::input);
+    }
+
     /// Graph capture and replay of steady-state runs (on by default).
     void set_graph_replay(bool on) {
         replay_enabled_ = on;
@@ -257,9 +336,125 @@ private:
         auto inst = meta_->at(path, kind == detail::IOKind::output ? "view_output" : "view_input");
         if (!dtype_compatible(inst->dtype, DT) || inst->rank != Rank)
             throw std::invalid_argument("naga::cospan: view: '" + std::string{path} + "' is " + dtype_name(inst->dtype) + " of rank " +
-                                        std::to_string(inst->rank));
+                                        std::to_string(inst->rank) + ": the wrong dtype or rank for this view");
+        if (inst->var->per_index || inst->var->hooked_ragged)
+            throw std::invalid_argument("naga::cospan: view: '" + std::string{path} +
+                                        "' is ragged and has no rectangular host view; use view_ragged_input / view_ragged_output");
         if (inst->var->pending) meta_->materialize(*inst->var, "view");
         return HostView<DT, Rank>(inst->var, &meta_->exec);
+    }
+    template <DType DT, int Rank>
+    HostRagged<DT, Rank> view_ragged(std::string_view path, detail::IOKind kind) {
+        auto inst = meta_->at(path, kind == detail::IOKind::output ? "view_ragged_output" : "view_ragged_input");
+        if (!dtype_compatible(inst->dtype, DT) || inst->rank != Rank)
+            throw std::invalid_argument("naga::cospan: view: '" + std::string{path} + "' is " + dtype_name(inst->dtype) + " of rank " +
+                                        std::to_string(inst->rank) + ": the wrong dtype or rank for this view");
+        if (inst->var->pending) meta_->materialize(*inst->var, "view");
+        return HostRagged<DT, Rank>(inst->var, &meta_->exec);
+    }
+
+    using RaggedSteps = std::vector<std::pair<std::size_t, detail::HookRecord*>>;
+
+    /// Where each ragged hook runs in the walk: directly after the last leaf writing its source, else before the first leaf
+    /// holding its target (else after every leaf). A holder of the target before that point is an error.
+    static RaggedSteps ragged_steps(detail::ModelMeta& m) {
+        RaggedSteps                                              out;
+        std::vector<std::pair<detail::VarMeta*, detail::IOKind>> pv;
+        for (detail::HookRecord& h : m.hooks) {
+            if (!h.ragged) continue;
+            const detail::VarMeta* t = h.inst->var.get();
+            const detail::VarMeta* s = h.source->var.get();
+            if (t == s) throw std::invalid_argument("naga::cospan: shape hook on '" + h.target + "': its source is its own target");
+            constexpr std::size_t none = ~std::size_t{0};
+            std::size_t           first_holder = none, last_writer = none;
+            for (std::size_t k = 0; k < m.leaves.size(); ++k) {
+                pv.clear();
+                m.leaves[k]->port_vars(pv);
+                for (const auto& [v, kind] : pv) {
+                    if (v == t && first_holder == none) first_holder = k;
+                    if (v == s && kind == detail::IOKind::output) last_writer = k;
+                }
+            }
+            const std::size_t at = last_writer != none ? last_writer + 1 : first_holder != none ? first_holder : m.leaves.size();
+            if (first_holder != none && first_holder < at)
+                throw std::invalid_argument("naga::cospan: shape hook on '" + h.target + "': its source '" + h.source_path + "' is written by '" +
+                                            m.leaves[last_writer]->path + "', which runs at or after '" + m.leaves[first_holder]->path +
+                                            "', a holder of '" + h.target + "'");
+            out.emplace_back(at, &h);
+        }
+        std::stable_sort(out.begin(), out.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        return out;
+    }
+
+    /// One eager forward leaf by leaf with the ragged hooks as steps between the leaves.
+    void run_stepwise(detail::ModelMeta& m, const RaggedSteps& steps) {
+        detail::Executor& ex = m.exec;
+        if (ex.has_graph()) {
+            ex.drop_graph();
+            graph_sig_.clear();
+        }
+        last_sig_.clear();
+        std::size_t si = 0;
+        for (std::size_t k = 0;; ++k) {
+            while (si < steps.size() && steps[si].first == k) ragged_step(m, *steps[si++].second);
+            if (k == m.leaves.size()) break;
+            const detail::LaunchRecord rec = m.leaves[k]->prepare(m);
+            ex.submit(*m.leaves[k], rec);
+        }
+    }
+
+    /// A counts_from / offsets_from hook (reference shape_hook.hpp:215-348, :425-473): the target's device column from the
+    /// source's values, one wait for [total, flags].
+    static void ragged_step(detail::ModelMeta& m, detail::HookRecord& h) {
+        detail::VarMeta&  src = *h.source->var;
+        detail::VarMeta&  tgt = *h.inst->var;
+        const std::string who = "the shape hook on '" + h.target + "'";
+        if (src.pending) m.materialize(src, who);
+        if (!src.shape_known() || (!src.has_value && !src.data))
+            throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' has no value yet (no source, no initial value)");
+        if (src.per_index) throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' is itself ragged");
+        const std::size_t count = src.dims[0];
+        const std::size_t k     = src.rank == 2 ? src.dims[1] : 1;
+        if (h.offsets_mode && count == 0)
+            throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' holds no offsets (it needs N + 1 >= 1 entries)");
+        if (!h.offsets_mode && k == 0) throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' has no column 0");
+        const std::size_t n = h.offsets_mode ? count - 1 : count;
+        if (tgt.fixed[0] && tgt.known[0] && tgt.dims[0] != n)
+            throw std::invalid_argument("naga::cospan: " + who + ": dimension 0 of '" + tgt.name + "' is fixed at " + std::to_string(tgt.dims[0]) +
+                                        " but '" + h.source_path + "' gives N = " + std::to_string(n));
+        int code = 0;
+        switch (src.dtype) {
+            case DType::i32: code = 0; break;
+            case DType::u32: code = 1; break;
+            case DType::i64: code = 2; break;
+            case DType::u64: code = 3; break;
+            default: throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' must have an integer dtype");
+        }
+        // every earlier launch (the source's writers, the readers of the last column) is complete
+        m.exec.synchronize();
+        const cs_hip_stream s = m.exec.stream(0);
+        if (!h.column) h.column = std::make_shared<detail::DeviceColumn>();
+        detail::DeviceColumn& c = *h.column;
+        c.reserve(n);
+        c.reserve_workspace(count, s);
+        detail::hip_check(cs_hip_build_column(code, src.data, count, k, h.offsets_mode ? 1 : 0, c.extents, c.offs, c.result, c.workspace, s),
+                          who);
+        std::uint32_t res[2] = {0, 0};
+        detail::hip_check(cs_hip_memcpy_async(res, c.result, sizeof res, s), who);
+        detail::hip_check(cs_hip_stream_synchronize(s), who);
+        if ((res[1] & CS_HIP_FLAG_NEGATIVE) == 0 && (res[1] & CS_HIP_FLAG_EXCEEDS32) != 0)
+            throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path +
+                                        "' holds a value or a total past 2^32 - 1; ragged totals are limited to 2^32 - 1 elements");
+        if (res[1] != 0)
+            throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path + "' holds " +
+                                        (h.offsets_mode ? std::string{"offsets that are negative, do not start at 0, or decrease"}
+                                                        : std::string{"a negative value"}) +
+                                        "; a size cannot be negative");
+        c.n     = n;
+        c.total = res[0];
+        ++c.version;
+        tgt.set_device_counts(h.column);
+        src.reads.clear();  // the build was the source's one read and it is complete
     }
 
     /// After a graph launch on stream 0: every variable's last event is the graph's completion.

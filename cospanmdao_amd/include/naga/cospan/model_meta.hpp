@@ -44,6 +44,8 @@ public:
     /// Materializes inputs, runs the user hooks, materializes outputs, points the wrappers and describes the launch.
     /// Fills `reads` and `writes` with the variables the launch touches.
     virtual LaunchRecord prepare(ModelMeta& m) = 0;
+    /// The variables of every port of the leaf and their kinds (known before any forward).
+    virtual void port_vars(std::vector<std::pair<VarMeta*, IOKind>>& out) const = 0;
     std::string           path;
     std::vector<VarMeta*> reads, writes;
 };
@@ -248,11 +250,18 @@ private:
     std::vector<TimedLaunch>* timing_ = nullptr;
 };
 
-/// A shape hook's run-time step: sets dimensions of its target before the walk (reference shape_hook.hpp).
+/// A shape hook's run-time step (reference shape_hook.hpp): a plain hook sets dimensions of its target before the walk;
+/// a ragged hook (counts_from / offsets_from) builds its target's device column from the VALUES of its source, so it runs
+/// as a step of the walk, after the source's producer and before the target's first user, with one host wait.
 struct HookRecord {
     std::string                               target;
     std::shared_ptr<Instance>                 inst;
     std::function<void(ModelMeta&, VarMeta&)> apply;
+    bool                                      ragged       = false;
+    bool                                      offsets_mode = false;
+    std::string                               source_path;
+    std::shared_ptr<Instance>                 source;
+    std::shared_ptr<DeviceColumn>             column;  ///< reused over forwards
 };
 
 struct ModelMeta {
@@ -292,6 +301,24 @@ struct ModelMeta {
                                         ", rank " + std::to_string(gone->rank) + ") cannot share a variable");
         if (keep->data && gone->data)
             throw std::logic_error("naga::cospan: " + who + ": merging two materialized variables is not defined (yet)");
+        // ragged definitions: declared counts are definitions and must agree
+        if (gone->counts_fixed && gone->counts_on_host) {
+            if (keep->counts_fixed && keep->counts_on_host && keep->counts != gone->counts) {
+                const bool n_differs = keep->counts.size() != gone->counts.size();
+                throw std::invalid_argument("naga::cospan: " + who + ": " +
+                                            (n_differs ? "dimension 0: '" + keep->name + "' declares " + std::to_string(keep->counts.size()) +
+                                                             " and '" + gone->name + "' declares " + std::to_string(gone->counts.size()) +
+                                                             ": the extents must match; "
+                                                       : std::string{}) +
+                                            "dimension 1: the counts must match: '" + keep->name + "' " + keep->counts_text() + ", '" +
+                                            gone->name + "' " + gone->counts_text());
+            }
+            if (!keep->counts_fixed) {
+                keep->set_host_counts(gone->counts, who);
+                keep->counts_fixed = true;
+                keep->fixed[0]     = true;
+            }
+        }
         for (int d = 0; d < keep->rank; ++d) {
             if (gone->fixed[d]) {
                 if (keep->fixed[d] && keep->dims[d] != gone->dims[d])
@@ -305,6 +332,8 @@ struct ModelMeta {
         }
         keep->has_producer = keep->has_producer || gone->has_producer;
         keep->scratch      = keep->scratch || gone->scratch;
+        keep->ragged_capable = keep->ragged_capable || gone->ragged_capable;
+        keep->hooked_ragged  = keep->hooked_ragged || gone->hooked_ragged;
         if (!keep->init && gone->init) {
             keep->init    = gone->init;
             keep->pending = gone->pending;

@@ -58,6 +58,19 @@ struct AsIs {};
 inline constexpr AsIs as_is{};
 }  // namespace dim
 
+/// Ragged shape-hook descriptors (reference shape_hook.hpp:79-85): the target's N and counts come from the VALUES of an
+/// integer variable: its counts (column 0 of a rank-2 [N, k] source) or its N + 1 offsets.
+namespace ragged {
+struct CountsFrom {
+    std::string path;
+};
+struct OffsetsFrom {
+    std::string path;
+};
+inline CountsFrom  counts_from(std::string path) { return CountsFrom{std::move(path)}; }
+inline OffsetsFrom offsets_from(std::string path) { return OffsetsFrom{std::move(path)}; }
+}  // namespace ragged
+
 class System;
 
 namespace detail {
@@ -137,7 +150,10 @@ public:
             throw std::invalid_argument("naga::cospan: connect: the destination '" + d->path + "' is not an input");
         if (d->var->has_producer && d->var != s->var)
             throw std::invalid_argument("naga::cospan: connect: '" + d->path + "' already has a source");
-        m.merge(s->var, d->var, "connect '" + s->path + "' -> '" + d->path + "'");
+        if (!d->ragged && ((s->ragged && s->kind == detail::IOKind::output) || s->var->per_index || s->var->hooked_ragged))
+            throw std::invalid_argument("naga::cospan: connect('" + s->path + "', '" + d->path + "'): '" + s->path +
+                                        "' is ragged; a dense Input cannot see per-index extents: declare '" + d->path + "' a RaggedInput");
+        m.merge(s->var, d->var, "connect('" + s->path + "', '" + d->path + "')");
     }
 
     /// An alias in this namespace for the child's variable (DT and Rank must match).
@@ -220,6 +236,13 @@ public:
         m.hooks.push_back(std::move(rec));
     }
 
+    /// A ragged hook: the target (a ragged port or its alias, rank 2) takes N and its per-index counts, every forward, from
+    /// the values of `source` (an integer variable of rank 1, or column 0 of rank 2): counts_from reads counts, offsets_from
+    /// N + 1 offsets. The hook runs after the source's producer, with one host wait for the total and the flags (negative
+    /// counts, totals past 2^32 - 1 are errors), as the reference's (shape_hook.hpp:425-473).
+    void shape_hook(std::string_view target, ragged::CountsFrom c) { ragged_hook(target, c.path, false); }
+    void shape_hook(std::string_view target, ragged::OffsetsFrom o) { ragged_hook(target, o.path, true); }
+
     /// The initial value of a free input (a child's input or an alias of this System).
     void init_input(std::string_view target, InitSpec init) {
         detail::ModelMeta& m    = meta("init_input");
@@ -231,6 +254,33 @@ public:
     }
 
 private:
+    void ragged_hook(std::string_view target, const std::string& source, bool offsets_mode) {
+        detail::ModelMeta& m    = meta("shape_hook");
+        auto               inst = m.at(detail::ModelMeta::join(path_, target), "shape_hook");
+        auto               src  = m.at(detail::ModelMeta::join(path_, source), "shape_hook");
+        if (inst->rank != 2 || !inst->ragged)
+            throw std::invalid_argument("naga::cospan: shape_hook: '" + inst->path + "' is not a ragged port (RaggedInput / RaggedOutput of rank 2)");
+        const DType d = src->dtype;
+        if (d != DType::u32 && d != DType::i32 && d != DType::u64 && d != DType::i64)
+            throw std::invalid_argument("naga::cospan: shape_hook: '" + src->path + "' is " + dtype_name(d) + "; counts come from an integer variable");
+        if (offsets_mode ? src->rank != 1 : (src->rank != 1 && src->rank != 2))
+            throw std::invalid_argument("naga::cospan: shape_hook: '" + src->path + "' has rank " + std::to_string(src->rank) +
+                                        (offsets_mode ? "; offsets come from a rank-1 variable" : "; counts come from rank 1 or column 0 of rank 2"));
+        detail::VarMeta& v = *inst->var;
+        if (v.counts_fixed)
+            throw std::invalid_argument("naga::cospan: shape_hook: the counts of '" + inst->path + "' are declared; a hook cannot own them");
+        v.hooked_ragged = v.counts_fixed = true;
+        detail::HookRecord rec;
+        rec.target       = inst->path;
+        rec.inst         = inst;
+        rec.apply        = [](detail::ModelMeta&, detail::VarMeta&) {};
+        rec.ragged       = true;
+        rec.offsets_mode = offsets_mode;
+        rec.source_path  = src->path;
+        rec.source       = src;
+        m.hooks.push_back(std::move(rec));
+    }
+
     friend struct detail::SystemAccess;
     detail::ModelMeta& meta(const char* what) const {
         if (!meta_) throw std::logic_error(std::string{"naga::cospan: "} + what + " is legal inside setup() only");

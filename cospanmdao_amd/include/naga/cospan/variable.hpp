@@ -173,6 +173,89 @@ struct Event {
 using EventPtr = std::shared_ptr<Event>;
 
 // -----------------------------------------------------------------------------
+// Ragged extents (reference variable.hpp:257-267, metadata.hpp:67-80)
+// -----------------------------------------------------------------------------
+
+/// The per-index extents of a ragged variable on the device: extents[n] and their exclusive scan offs[n + 1] (u32, as
+/// the reference's device column), and the host copy of the total. Built by a ragged shape hook (cs_hip_build_column) or
+/// uploaded from host counts.
+struct DeviceColumn {
+    std::uint32_t* extents   = nullptr;
+    std::uint32_t* offs      = nullptr;
+    std::uint32_t* result    = nullptr;  ///< [total, flag] of a column build
+    void*          workspace = nullptr;  ///< the scan's look-back words (zeroed once; every scan leaves it zeroed)
+    std::size_t    cap_n = 0, ws_bytes = 0;
+    std::size_t    n     = 0;
+    std::uint64_t  total = 0, version = 0;
+
+    DeviceColumn() = default;
+    DeviceColumn(const DeviceColumn&)            = delete;
+    DeviceColumn& operator=(const DeviceColumn&) = delete;
+    ~DeviceColumn() {
+        for (void* p : {static_cast<void*>(extents), static_cast<void*>(offs), static_cast<void*>(result), workspace})
+            if (p) cs_hip_free(p);
+    }
+    /// Room for n indices (the contents are not kept).
+    void reserve(std::size_t want) {
+        if (want <= cap_n && extents) return;
+        for (void** p : {reinterpret_cast<void**>(&extents), reinterpret_cast<void**>(&offs), reinterpret_cast<void**>(&result)})
+            if (*p) {
+                cs_hip_free(*p);
+                *p = nullptr;
+            }
+        const std::size_t c = want < 64 ? 64 : want;
+        hip_check(cs_hip_malloc(reinterpret_cast<void**>(&extents), c * 4), "ragged column");
+        hip_check(cs_hip_malloc(reinterpret_cast<void**>(&offs), (c + 1) * 4), "ragged column");
+        hip_check(cs_hip_malloc(reinterpret_cast<void**>(&result), 16), "ragged column");
+        cap_n = c;
+    }
+    /// A zeroed scan workspace for a column build over `count` source entries.
+    void reserve_workspace(std::size_t count, cs_hip_stream s) {
+        const std::size_t need = cs_hip_scan_workspace_bytes(count);
+        if (need <= ws_bytes && workspace) return;
+        if (workspace) cs_hip_free(workspace);
+        workspace = nullptr;
+        hip_check(cs_hip_malloc(&workspace, need), "ragged column workspace");
+        hip_check(cs_hip_scan_workspace_init(workspace, need, s), "ragged column workspace");
+        ws_bytes = need;
+    }
+};
+
+}  // namespace detail
+
+/// A host array of per-index counts for a ragged declaration (its length is N): cs::counts(v).
+struct Counts {
+    std::vector<std::size_t> v;
+};
+inline Counts counts(std::vector<std::size_t> v) { return Counts{std::move(v)}; }
+/// The same as N + 1 offsets: cs::offsets(o), o[0] == 0, non-decreasing.
+struct Offsets {
+    std::vector<std::size_t> v;
+};
+inline Offsets offsets(std::vector<std::size_t> v) { return Offsets{std::move(v)}; }
+
+/// The shape of a ragged port (N indices and one count per index): what shape_of returns for a RaggedInput /
+/// RaggedOutput and what set_shape takes. The counts are on the host, or, for a hooked ragged variable, on the device
+/// (`column`): they come down to the host only when asked for (counts_on_host).
+struct RaggedShape {
+    std::size_t                                 n     = 0;
+    std::uint64_t                               total = 0;
+    std::vector<std::size_t>                    counts;  ///< the host counts (empty when `column` is set)
+    std::shared_ptr<const detail::DeviceColumn> column;
+    bool                                        on_device() const noexcept { return column != nullptr; }
+    /// The counts on the host (the device column copied down, waiting for it).
+    std::vector<std::size_t> counts_on_host() const {
+        if (!column) return counts;
+        std::vector<std::uint32_t> e(n);
+        if (n) detail::hip_check(cs_hip_memcpy_async(e.data(), column->extents, n * 4, nullptr), "ragged counts");
+        detail::hip_check(cs_hip_stream_synchronize(nullptr), "ragged counts");
+        return std::vector<std::size_t>(e.begin(), e.end());
+    }
+};
+
+namespace detail {
+
+// -----------------------------------------------------------------------------
 // Variables
 // -----------------------------------------------------------------------------
 
@@ -195,6 +278,18 @@ struct VarMeta {
     void*       data     = nullptr;
     std::size_t capacity = 0;  ///< bytes allocated
 
+    // ---- ragged (rank 2: dimension 1 holds one extent per index) ----
+    bool                          ragged_capable = false;  ///< a RaggedInput / RaggedOutput names it
+    bool                          per_index      = false;  ///< dimension 1 is per index: the variable is ragged
+    bool                          counts_fixed   = false;  ///< the counts come from a definition or a hook
+    bool                          hooked_ragged  = false;  ///< a counts_from / offsets_from hook owns N and the counts
+    bool                          counts_on_host = false;  ///< `counts` holds them (else only `column` does)
+    std::vector<std::size_t>      counts;
+    std::shared_ptr<DeviceColumn> column;
+    std::uint64_t                 total          = 0;
+    std::uint64_t                 counts_version = 0;  ///< bumped at every change of the host counts
+    std::uint64_t                 uploaded       = ~std::uint64_t{0};  ///< the counts_version `column` was uploaded from
+
     EventPtr              write;  ///< the last writer
     std::vector<EventPtr> reads;  ///< readers since the last write
 
@@ -206,17 +301,82 @@ struct VarMeta {
     }
 
     bool shape_known() const {
+        if (per_index) return known[0] && (counts_on_host || column != nullptr);
         for (int d = 0; d < rank; ++d)
             if (!known[d]) return false;
         return true;
     }
     std::size_t elements() const {
+        if (per_index) return static_cast<std::size_t>(total);
         std::size_t n = 1;
         for (int d = 0; d < rank; ++d) n *= dims[d];
         return n;
     }
+    /// Makes the variable ragged with host counts (N = counts.size()).
+    void set_host_counts(std::vector<std::size_t> c, const std::string& who) {
+        if (rank != 2) throw std::invalid_argument("naga::cospan: " + who + ": '" + name + "' is not rank 2; a ragged variable is");
+        std::uint64_t t = 0;
+        for (std::size_t e : c) t += e;
+        if (t > 0xffffffffull)
+            throw std::invalid_argument("naga::cospan: " + who + ": '" + name + "': ragged totals are limited to 2^32 - 1 elements");
+        if (fixed[0] && known[0] && dims[0] != c.size())
+            throw std::invalid_argument("naga::cospan: " + who + ": dimension 0 of '" + name + "' is fixed at " + std::to_string(dims[0]) +
+                                        "; counts of length " + std::to_string(c.size()) + " do not fit");
+        const bool same = per_index && counts_on_host && counts == c;
+        per_index      = true;
+        counts_on_host = true;
+        dims[0]        = c.size();
+        dims[1]        = 0;
+        known[0] = known[1] = true;
+        total               = t;
+        if (!same) {
+            counts = std::move(c);
+            ++counts_version;
+            column.reset();
+        }
+    }
+    /// Makes the variable ragged with the counts of a device column (a hook's or another variable's).
+    void set_device_counts(std::shared_ptr<DeviceColumn> col) {
+        per_index      = true;
+        counts_on_host = false;
+        counts.clear();
+        dims[0]  = col->n;
+        dims[1]  = 0;
+        known[0] = known[1] = true;
+        total               = col->total;
+        column              = std::move(col);
+        uploaded            = counts_version;
+    }
+    /// The device column of a ragged variable with host counts: uploaded when the counts changed (waits for the copy).
+    void ensure_device_counts(cs_hip_stream s) {
+        if (!per_index || !counts_on_host || (column && uploaded == counts_version)) return;
+        const std::size_t          n = counts.size();
+        std::vector<std::uint32_t> e(n), o(n + 1);
+        std::uint64_t              run = 0;
+        for (std::size_t i = 0; i < n; ++i) {
+            e[i] = static_cast<std::uint32_t>(counts[i]);
+            o[i] = static_cast<std::uint32_t>(run);
+            run += counts[i];
+        }
+        o[n] = static_cast<std::uint32_t>(run);
+        // a new column: kernels of earlier forwards may still read the old one (it lives until its last holder lets go)
+        column = std::make_shared<DeviceColumn>();
+        column->reserve(n);
+        column->n     = n;
+        column->total = run;
+        if (n) hip_check(cs_hip_memcpy_async(column->extents, e.data(), n * 4, s), "ragged counts upload");
+        hip_check(cs_hip_memcpy_async(column->offs, o.data(), (n + 1) * 4, s), "ragged counts upload");
+        hip_check(cs_hip_stream_synchronize(s), "ragged counts upload");
+        uploaded = counts_version;
+    }
+    std::string counts_text() const {
+        std::string t = "[";
+        for (std::size_t i = 0; i < counts.size(); ++i) t += (i ? "," : "") + std::to_string(counts[i]);
+        return t + "]";
+    }
     std::size_t bytes() const { return elements() * dtype_size(dtype); }
     std::string shape_text() const {
+        if (per_index) return "(" + std::to_string(dims[0]) + ", per index)";
         std::string s = "(";
         for (int d = 0; d < rank; ++d) s += (d ? ", " : "") + (known[d] ? std::to_string(dims[d]) : std::string{"?"});
         return s + ")";
@@ -257,6 +417,7 @@ struct Instance {
     IOKind                   kind  = IOKind::input;
     DType                    dtype = DType::f32;
     int                      rank  = 1;
+    bool                     ragged = false;  ///< a RaggedInput / RaggedOutput port (or an alias of one)
     std::shared_ptr<VarMeta> var;
 };
 

@@ -165,9 +165,125 @@ private:
     friend struct detail::WrapperAccess;
 };
 
+// -----------------------------------------------------------------------------
+// Ragged ports (reference wrappers.hpp:244-354): rank 2 only, N blocks, block i of extent_1(i) elements
+// -----------------------------------------------------------------------------
+
+/// One block of a ragged port: a rank-R view of contiguous elements.
+template <class T, int R>
+struct Block {
+    T*       data_  = nullptr;
+    range<R> shape_ = detail::filled_range<R>(0);
+
+    CS_HD T& operator[](id<R> idx) const {
+        std::size_t o = 0;
+        for (int d = 0; d < R; ++d) o = o * shape_[d] + idx[d];
+        return data_[o];
+    }
+    CS_HD T&          operator[](std::size_t flat) const { return data_[flat]; }
+    CS_HD range<R>    shape() const { return shape_; }
+    CS_HD std::size_t size() const { return shape_.size(); }
+    CS_HD T*          get_pointer() const { return data_; }
+};
+
+/// The block of an ATOMIC ragged output: operator[] gives the relaxed, agent-scope atomic reference a dense atomic Output
+/// gives; no bare reference.
+template <class T, int R>
+struct AtomicBlock {
+    Block<T, R>                 b_{};
+    CS_HD atomic_ref<T>         operator[](id<R> idx) const { return atomic_ref<T>{&b_[idx]}; }
+    CS_HD atomic_ref<T>         operator[](std::size_t flat) const { return atomic_ref<T>{&b_[flat]}; }
+    CS_HD range<R>              shape() const { return b_.shape(); }
+    CS_HD std::size_t           size() const { return b_.size(); }
+};
+
+namespace detail {
+/// Block i of a ragged port: with per-index extents (`per`, `offsets` on the device) block i holds per[i] elements from
+/// offsets[i]; a dense variable read through a ragged port has no offsets and block i is its row i (`ext` elements).
+template <class T>
+CS_HD Block<T, 1> ragged_block(T* data, const std::uint32_t* offsets, const std::uint32_t* per, std::size_t ext, std::size_t i) {
+    Block<T, 1> b;
+    if (per != nullptr) {
+        b.shape_[0] = per[i];
+        b.data_     = data + offsets[i];
+    } else {
+        b.shape_[0] = ext;
+        b.data_     = data + i * ext;
+    }
+    return b;
+}
+}  // namespace detail
+
+/// RaggedInput<DT, 2>: a read-only ragged port; also reads a DENSE rank-2 variable (block i = row i).
+template <DType DT, int Rank>
+    requires(!is_atomic_v<DT>)
+struct RaggedInput {
+    static_assert(Rank == 2, "cospan: a ragged port is rank 2 only");
+    using value_type              = cpp_type_t<DT>;
+    static constexpr DType dtype  = DT;
+    static constexpr int   rank   = Rank;
+    static constexpr bool  ragged = true;
+
+    CS_HD std::size_t size() const { return n_; }
+    CS_HD Block<const value_type, 1> block(std::size_t i) const {
+        return detail::ragged_block<const value_type>(data_, offsets_, per_, ext_, i);
+    }
+
+private:
+    std::size_t          n_       = 0;
+    const value_type*    data_    = nullptr;
+    const std::uint32_t* offsets_ = nullptr;
+    const std::uint32_t* per_     = nullptr;
+    std::size_t          ext_     = 0;
+    friend struct detail::WrapperAccess;
+};
+
+/// RaggedOutput<DT, 2>: the writable twin; with an atomic DT, block(i) is an AtomicBlock.
+template <DType DT, int Rank>
+struct RaggedOutput {
+    static_assert(Rank == 2, "cospan: a ragged port is rank 2 only");
+    using value_type              = cpp_type_t<DT>;
+    static constexpr DType dtype  = DT;
+    static constexpr int   rank   = Rank;
+    static constexpr bool  ragged = true;
+
+    CS_HD std::size_t size() const { return n_; }
+    CS_HD Block<value_type, 1> block(std::size_t i) const
+        requires(!is_atomic_v<DT>)
+    {
+        return detail::ragged_block<value_type>(data_, offsets_, per_, ext_, i);
+    }
+    CS_HD AtomicBlock<value_type, 1> block(std::size_t i) const
+        requires(is_atomic_v<DT>)
+    {
+        return AtomicBlock<value_type, 1>{detail::ragged_block<value_type>(data_, offsets_, per_, ext_, i)};
+    }
+
+private:
+    std::size_t          n_       = 0;
+    value_type*          data_    = nullptr;
+    const std::uint32_t* offsets_ = nullptr;
+    const std::uint32_t* per_     = nullptr;
+    std::size_t          ext_     = 0;
+    friend struct detail::WrapperAccess;
+};
+
+static_assert(std::is_trivially_copyable_v<RaggedInput<DType::f32, 2>>);
+static_assert(std::is_trivially_copyable_v<RaggedOutput<DType::f32, 2>>);
+
 namespace detail {
 /// Points a wrapper at its data (the executor's side; user code never does this).
 struct WrapperAccess {
+    /// A ragged port: n blocks; per-index extents and offsets on the device, or (both null) rows of `ext` elements.
+    template <class W>
+    static void point_ragged(W& w, void* data, std::size_t n, const std::uint32_t* offsets, const std::uint32_t* per, std::size_t ext) {
+        using T     = typename W::value_type;
+        w.n_        = n;
+        w.data_     = static_cast<T*>(data);
+        w.offsets_  = offsets;
+        w.per_      = per;
+        w.ext_      = ext;
+    }
     template <class W>
     static void point(W& w, void* data, const std::size_t* shape, const std::size_t* stride) {
         using T = typename W::value_type;

@@ -417,10 +417,19 @@ int    cs_hip_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, vo
 }
 int cs_hip_build_column(int dt, const void* src, size_t count, size_t k, int offsets_mode, uint32_t* ext, uint32_t* offs, uint32_t* result,
                         void* ws, cs_hip_stream s) {
-    if (offsets_mode) return fail(CS_HIP_ERR_INVALID, "mock: offsets mode is not emulated");
     if (!offs || !result || (count && (!src || !ext))) return fail(CS_HIP_ERR_INVALID, "cs_hip_build_column: null pointer");
     S().stats.scans++;
     const size_t wsb = cs_hip_scan_workspace_bytes(count);
+    if (offsets_mode) {
+        if (count == 0) return fail(CS_HIP_ERR_INVALID, "cs_hip_build_column: an offsets source holds no offsets");
+        enqueue(s, [=] {
+            if (ws && !all_zero(ws, wsb)) note_failure("build_column: the workspace is not zero when the scan starts");
+            uint32_t total = 0;
+            result[1]      = or_build_column(dt, src, count, 1, 1, ext, offs, &total);
+            result[0]      = total;
+        });
+        return CS_HIP_OK;
+    }
     enqueue(s, [=] {
         if (ws && !all_zero(ws, wsb)) note_failure("build_column: the workspace is not zero when the scan starts");
         switch (dt) {

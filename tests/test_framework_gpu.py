@@ -18,3 +18,14 @@ def test_fused_groups_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+
+
+def test_ragged_program():
+    """Ragged ports through cs::Model on the device: tests/framework/ragged.cpp (counts_from / offsets_from hooks, declared
+    counts, dense sources read as rows, a ragged port in a fused group, the error paths), every value checked."""
+    exe = os.path.join(HERE, "framework", "_build", "ragged")
+    assert os.path.exists(exe), "tests/framework/_build/ragged is not built (make -C tests/framework)"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]

@@ -40,3 +40,11 @@ def test_chain_host_logic():
     out = run("chain_host")
     for style, launches in (("unfused", 11), ("parallel", 11), ("fused", 8), ("onepass", 9), ("naive", 8), ("crafted", 8)):
         assert f"{style}: {launches} launches per chain" in out
+
+
+def test_ragged_host_logic():
+    """Ragged ports on the CPU (tests/framework/ragged.cpp built against the mock boundary): column builds from counts and
+    offsets, block offsets in every view, hook placement, declared counts merged at connect, and every error message."""
+    build()
+    out = run("ragged_host")
+    assert "offsets_from" in out and "fused group" in out
