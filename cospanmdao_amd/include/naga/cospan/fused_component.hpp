@@ -5,7 +5,7 @@
 // =============================================================================
 //
 // The reference's helpers (cospanmdao/include/naga/cospan/fused_component.hpp:455-569) with the same meaning:
-//   cs::fuse_components<opts>(name, [connects_desc(...)], items...)   a leaf whose stages run in one kernel
+//   cs::fuse_components<opts>(name, [connects_desc(...)], [ties_desc(...)], items...)   one kernel of stages
 //   cs::segment<cs::local_mem::shared|partitioned, cs::range_mode::whole|own>(descs...)
 //   cs::global_barrier()                                               every group of the next segment waits for every
 //                                                                      group of the earlier segments
@@ -102,7 +102,7 @@ class FusedGroup : public FusedBase {
     static constexpr int N = L::N;
 
 public:
-    explicit FusedGroup(ConnectsDesc c, Items... items) : connects_{std::move(c)}, items_{std::move(items)...} {}
+    FusedGroup(ConnectsDesc c, TiesDesc t, Items... items) : connects_{std::move(c)}, ties_{std::move(t)}, items_{std::move(items)...} {}
 
     void build(ModelMeta& m, const std::string& path) override {
         meta_ = &m;
@@ -111,6 +111,8 @@ public:
         Boundary boundary = Boundary::none;
         std::apply([&](auto&... item) { (add_item(item, k, boundary), ...); }, items_);
         if (k != N) throw std::logic_error("naga::cospan: fused group '" + path + "': stage count mismatch");
+        // ties first: a connection may name a tie alias as its source
+        for (const TieDesc& t : ties_.ties) apply_tie(m, path, t);
         for (auto& [src, dst] : connects_.pairs) {
             auto s = m.at(ModelMeta::join(path, src), "connects_desc");
             auto d = m.at(ModelMeta::join(path, dst), "connects_desc");
@@ -195,6 +197,7 @@ private:
     }
 
     ConnectsDesc             connects_;
+    TiesDesc                 ties_;
     std::tuple<Items...>     items_;
     typename L::comps        comps_;
     std::vector<SegmentPlan> segs_;
@@ -229,17 +232,30 @@ auto segment(Descs... d) {
 inline detail::BoundaryItem<detail::Boundary::global_barrier> global_barrier() { return {}; }
 
 /// A fused group of the given items, for add_subsystem (reference fused_component.hpp:528-569).
+/// Optional descriptors, in this order: cs::connects_desc(...), cs::ties_desc(...).
 template <class... Opts, class... Items>
-auto fuse_components(std::string name, ConnectsDesc c, Items... items) {
+    requires(sizeof...(Items) >= 1)
+auto fuse_components(std::string name, ConnectsDesc c, TiesDesc t, Items... items) {
     static_assert((detail::is_fuse_option_v<Opts> && ...), "naga::cospan: the fuse_components options are cs::group_waits_enabled and cs::range_split::*");
     constexpr bool          waits = detail::option_count_v<group_waits_enabled, Opts...> > 0;
     constexpr detail::Split sp    = detail::split_of<Opts...>();
-    return subsystem_desc<detail::FusedGroup<waits, sp, Items...>>(std::move(name), std::move(c), std::move(items)...);
+    return subsystem_desc<detail::FusedGroup<waits, sp, Items...>>(std::move(name), std::move(c), std::move(t), std::move(items)...);
 }
 template <class... Opts, class... Items>
-    requires(sizeof...(Items) >= 1 && !std::is_same_v<std::tuple_element_t<0, std::tuple<Items...>>, ConnectsDesc>)
+    requires(sizeof...(Items) >= 1 && !std::is_same_v<std::tuple_element_t<0, std::tuple<Items...>>, TiesDesc>)
+auto fuse_components(std::string name, ConnectsDesc c, Items... items) {
+    return fuse_components<Opts...>(std::move(name), std::move(c), TiesDesc{}, std::move(items)...);
+}
+template <class... Opts, class... Items>
+    requires(sizeof...(Items) >= 1)
+auto fuse_components(std::string name, TiesDesc t, Items... items) {
+    return fuse_components<Opts...>(std::move(name), ConnectsDesc{}, std::move(t), std::move(items)...);
+}
+template <class... Opts, class... Items>
+    requires(sizeof...(Items) >= 1 && !std::is_same_v<std::tuple_element_t<0, std::tuple<Items...>>, ConnectsDesc> &&
+             !std::is_same_v<std::tuple_element_t<0, std::tuple<Items...>>, TiesDesc>)
 auto fuse_components(std::string name, Items... items) {
-    return fuse_components<Opts...>(std::move(name), ConnectsDesc{}, std::move(items)...);
+    return fuse_components<Opts...>(std::move(name), ConnectsDesc{}, TiesDesc{}, std::move(items)...);
 }
 
 }  // namespace naga::cospan

@@ -48,6 +48,82 @@ struct ConnectsDesc {
 };
 inline ConnectsDesc connects_desc(std::initializer_list<std::pair<std::string, std::string>> pairs) { return ConnectsDesc{pairs}; }
 
+/// Whether tied outputs must be atomic (reference system.hpp:178-182): several writers of one atomic variable accumulate in
+/// any order; plain writers (allow_plain) are sequenced by stage order in a fused group and by launch order across leaves.
+enum class TieOutputsPolicy { require_atomic, allow_plain };
+
+/// One tie as a value: the members (inputs, or outputs) become one variable that the alias names (reference
+/// fused_component.hpp:500-515, cs::tie_inputs / cs::tie_outputs inside cs::ties_desc).
+struct TieDesc {
+    bool                     outputs = false;
+    DType                    dtype   = DType::f32;
+    int                      rank    = 1;
+    std::vector<std::string> members;
+    std::string              alias;
+    ShapeSpec                shape;
+    TieOutputsPolicy         policy = TieOutputsPolicy::require_atomic;
+};
+struct TiesDesc {
+    std::vector<TieDesc> ties;
+};
+inline TiesDesc ties_desc(std::initializer_list<TieDesc> t) { return TiesDesc{t}; }
+template <DType DT, int Rank>
+TieDesc tie_inputs(std::initializer_list<std::string_view> members, std::string alias, ShapeSpec shape = {}) {
+    TieDesc t;
+    t.dtype = DT;
+    t.rank  = Rank;
+    for (std::string_view m : members) t.members.emplace_back(m);
+    t.alias = std::move(alias);
+    t.shape = shape;
+    return t;
+}
+template <DType DT, int Rank>
+TieDesc tie_outputs(std::initializer_list<std::string_view> members, std::string alias, ShapeSpec shape = {},
+                    TieOutputsPolicy policy = TieOutputsPolicy::require_atomic) {
+    TieDesc t = tie_inputs<DT, Rank>(members, std::move(alias), shape);
+    t.outputs = true;
+    t.policy  = policy;
+    return t;
+}
+
+namespace detail {
+/// Makes the alias's variable (base path + alias) and merges every member into it.
+inline void apply_tie(ModelMeta& m, const std::string& base, const TieDesc& t) {
+    const char* what = t.outputs ? "tie_outputs" : "tie_inputs";
+    const std::string full = ModelMeta::join(base, t.alias);
+    if (t.members.size() < 2) throw std::invalid_argument(std::string{"naga::cospan: "} + what + " '" + full + "' needs at least two members");
+    if (t.outputs && t.policy == TieOutputsPolicy::require_atomic && !is_atomic(t.dtype))
+        throw std::invalid_argument(std::string{"naga::cospan: tie_outputs '"} + full + "': " + dtype_name(t.dtype) +
+                                    " is not atomic: plain writers of one variable need TieOutputsPolicy::allow_plain (they are then "
+                                    "sequenced by stage order)");
+    auto v   = std::make_shared<VarMeta>();
+    v->name  = full;
+    v->dtype = t.dtype;
+    v->rank  = t.rank;
+    for (int d = 0; d < t.shape.rank; ++d)
+        if (t.shape.specified[d]) {
+            v->dims[d]  = t.shape.extents[d];
+            v->known[d] = v->fixed[d] = true;
+        }
+    auto a   = std::make_shared<Instance>();
+    a->path  = full;
+    a->kind  = t.outputs ? IOKind::output : IOKind::input;
+    a->dtype = t.dtype;
+    a->rank  = t.rank;
+    a->var   = v;
+    m.add(a);
+    for (const std::string& mp : t.members) {
+        auto inst = m.at(ModelMeta::join(base, mp), what);
+        if (inst->kind != a->kind)
+            throw std::invalid_argument(std::string{"naga::cospan: "} + what + " '" + full + "': '" + inst->path + "' is not an " +
+                                        (t.outputs ? "output" : "input"));
+        if (!t.outputs && inst->var->has_producer)
+            throw std::invalid_argument(std::string{"naga::cospan: tie_inputs '"} + full + "': '" + inst->path + "' is fed by a connection");
+        m.merge(a->var, inst->var, std::string{what} + " '" + full + "'");
+    }
+}
+}  // namespace detail
+
 /// Shape-hook dimension descriptors (reference shape_hook.hpp: cs::dim::fixed_at_run, as_is).
 namespace dim {
 struct FixedAtRun {
@@ -172,30 +248,14 @@ public:
     /// A new variable every member (inputs of dtype DT and rank Rank) merges into; the alias names it.
     template <DType DT, int Rank>
     void tie_inputs(std::initializer_list<std::string_view> members, std::string name, ShapeSpec shape = {}) {
-        detail::ModelMeta& m = meta("tie_inputs");
-        if (members.size() < 2) throw std::invalid_argument("naga::cospan: tie_inputs needs at least two members");
-        auto v   = std::make_shared<detail::VarMeta>();
-        v->name  = detail::ModelMeta::join(path_, name);
-        v->dtype = DT;
-        v->rank  = Rank;
-        for (int d = 0; d < shape.rank; ++d)
-            if (shape.specified[d]) {
-                v->dims[d]  = shape.extents[d];
-                v->known[d] = v->fixed[d] = true;
-            }
-        auto a   = std::make_shared<detail::Instance>();
-        a->path  = v->name;
-        a->kind  = detail::IOKind::input;
-        a->dtype = DT;
-        a->rank  = Rank;
-        a->var   = v;
-        m.add(a);
-        for (std::string_view mp : members) {
-            auto inst = m.at(detail::ModelMeta::join(path_, mp), "tie_inputs");
-            if (inst->kind != detail::IOKind::input)
-                throw std::invalid_argument("naga::cospan: tie_inputs: '" + inst->path + "' is not an input");
-            m.merge(a->var, inst->var, "tie_inputs '" + v->name + "'");
-        }
+        detail::apply_tie(meta("tie_inputs"), path_, naga::cospan::tie_inputs<DT, Rank>(members, std::move(name), shape));
+    }
+    /// The same for outputs (reference system.hpp:178-182): several writers of one variable. Atomic dtypes accumulate in any
+    /// order; plain ones need TieOutputsPolicy::allow_plain and are sequenced by launch order.
+    template <DType DT, int Rank>
+    void tie_outputs(std::initializer_list<std::string_view> members, std::string name, ShapeSpec shape = {},
+                     TieOutputsPolicy policy = TieOutputsPolicy::require_atomic) {
+        detail::apply_tie(meta("tie_outputs"), path_, naga::cospan::tie_outputs<DT, Rank>(members, std::move(name), shape, policy));
     }
 
     /// Dimensions of `target` set at the start of every run (cs::dim::fixed_at_run(n), cs::dim::as_is).

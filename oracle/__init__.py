@@ -56,6 +56,13 @@ def lib():
             "or_ns_fields": (None, [C.c_uint32, C.c_int, C.c_int, _P, _P, _P]),
             "or_ns_residual_f64": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P, _P, _P, _P]),
             "or_ns_compare": (None, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "or_nsc_residual_f64": (None, [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, _P]),
+            "or_nsc_prim_f64": (None, [_P, _P]),
+            "or_nsc_slope_f64": (C.c_double, [C.c_double, C.c_double, C.c_int]),
+            "or_nsc_hllc_f64": (None, [_P, _P, C.c_int, _P]),
+            "or_nsc_visc_f64": (None, [_P, _P, _P, _P, C.c_int, _P]),
+            "or_nsc_cell_diag_f64": (None, [_P, _P, _P]),
+            "or_nsc_init_f32": (None, [C.c_int, C.c_int, C.c_float, _P]),
             "or_threads": (C.c_int, []),
         }
         for name, (res, args) in sig.items():
@@ -202,3 +209,57 @@ def build_column(src, offsets_mode=False, k=1):
     total = C.c_uint32(0)
     flag = lib().or_build_column(dt, ptr(src), count, k, 1 if offsets_mode else 0, ptr(ext), ptr(offs), C.byref(total))
     return ext, offs, total.value, flag
+
+
+# ---- compressible NS residual (examples/ns-residual/physics.hpp, reference.hpp) ----------------------------------------
+NSC_GAMMA, NSC_MU, NSC_PR, NSC_CFL = 1.4, 0.01, 0.72, 0.5
+
+
+def _d(a, n):
+    a = np.ascontiguousarray(a, np.float64)
+    assert a.size == n
+    return a
+
+
+def nsc_init(nx, ny, amp=1.0):
+    U = np.empty((4, ny, nx), np.float32)
+    lib().or_nsc_init_f32(nx, ny, amp, ptr(U))
+    return U
+
+
+def nsc_residual(U, walls=True, vanleer=False):
+    """(R [4, ny, nx] f64, diag {dt, ws, sq[4]}) of the FP64 restatement."""
+    U = np.ascontiguousarray(U, np.float32)
+    _, ny, nx = U.shape
+    R = np.empty((4, ny, nx))
+    d = np.empty(6)
+    lib().or_nsc_residual_f64(nx, ny, 1 if walls else 0, 1 if vanleer else 0, ptr(U), ptr(R), ptr(d))
+    return R, {"dt": d[0], "ws": d[1], "sq": d[2:6].copy()}
+
+
+def nsc_prim(U):
+    W = np.empty(4)
+    lib().or_nsc_prim_f64(ptr(_d(U, 4)), ptr(W))
+    return W
+
+
+def nsc_slope(a, b, vanleer=False):
+    return lib().or_nsc_slope_f64(float(a), float(b), 1 if vanleer else 0)
+
+
+def nsc_hllc(wl, wr, dir):
+    F = np.empty(4)
+    lib().or_nsc_hllc_f64(ptr(_d(wl, 4)), ptr(_d(wr, 4)), dir, ptr(F))
+    return F
+
+
+def nsc_visc(wl, wr, gl, gr, dir):
+    F = np.empty(4)
+    lib().or_nsc_visc_f64(ptr(_d(wl, 4)), ptr(_d(wr, 4)), ptr(_d(gl, 6)), ptr(_d(gr, 6)), dir, ptr(F))
+    return F
+
+
+def nsc_cell_diag(W):
+    dt, ws = C.c_double(), C.c_double()
+    lib().or_nsc_cell_diag_f64(ptr(_d(W, 4)), C.byref(dt), C.byref(ws))
+    return dt.value, ws.value

@@ -708,3 +708,207 @@ void or_ns_compare(int nx, int ny, double h, double nu, int store, const void* u
         sq[2] = d2;
     }
 }
+
+/* ---- compressible NS residual, FP64: examples/ns-residual/physics.hpp (numerics), reference.hpp:29-128 (assembly) --
+ * Conserved U = (rho, rho u, rho v, E), ideal gas gamma 1.4, mu 0.01, Pr 0.72, dx = dy = 1, arrays (v, y, x).
+ * R = -div F_conv + div F_visc per cell; the x-face i of row j is the west face of cell (j, i), the y-face j of column i
+ * its south face. x is periodic; y is periodic or has no-slip adiabatic walls (ghost rows mirror the interior with both
+ * momenta negated). MUSCL on primitives (minmod or van Leer), HLLC with Davis speeds, viscous flux with the normal
+ * derivative from the face's two cells and the tangential one from the mean of their central-difference gradients.
+ * Diagnostics: min over cells of 0.5 / (|u| + |v| + 2c + 4 nu_max), max of max(|u|, |v|) + c, sum of R^2 per variable. */
+#define NSC_GAMMA 1.4
+#define NSC_MU 0.01
+#define NSC_PR 0.72
+#define NSC_CFL 0.5
+
+void or_nsc_prim_f64(const double* U, double* W) {
+    const double r = U[0], u = U[1] / r, v = U[2] / r;
+    W[0] = r;
+    W[1] = u;
+    W[2] = v;
+    W[3] = (NSC_GAMMA - 1.0) * (U[3] - 0.5 * r * (u * u + v * v));
+}
+
+double or_nsc_slope_f64(double a, double b, int vanleer) {
+    if (a * b <= 0.0) return 0.0;
+    if (vanleer) return 2.0 * a * b / (a + b);
+    return a > 0.0 ? (a < b ? a : b) : (a > b ? a : b);
+}
+
+/* The physical flux and the HLLC flux of a face with normal x (dir 0) or y (dir 1); w = (rho, u, v, p). */
+static void nsc_euler_flux(const double* w, int dir, double* f) {
+    const double un = dir == 0 ? w[1] : w[2], ut = dir == 0 ? w[2] : w[1];
+    const double E = w[3] / (NSC_GAMMA - 1.0) + 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]);
+    f[0] = w[0] * un;
+    f[1] = w[0] * un * un + w[3];
+    f[2] = w[0] * un * ut;
+    f[3] = (E + w[3]) * un;
+}
+
+void or_nsc_hllc_f64(const double* wl, const double* wr, int dir, double* F) {
+    const double g = NSC_GAMMA;
+    const double ul = dir == 0 ? wl[1] : wl[2], vl = dir == 0 ? wl[2] : wl[1];
+    const double ur = dir == 0 ? wr[1] : wr[2], vr = dir == 0 ? wr[2] : wr[1];
+    const double cl = sqrt(g * wl[3] / wl[0]), cr = sqrt(g * wr[3] / wr[0]);
+    const double sl = fmin(ul - cl, ur - cr), sr = fmax(ul + cl, ur + cr); /* Davis */
+    const double ml = wl[0] * (sl - ul), mr = wr[0] * (sr - ur);          /* mass flux into each star region */
+    const double sm = (wr[3] - wl[3] + ul * ml - ur * mr) / (ml - mr);
+    double       f[4]; /* face frame: mass, normal momentum, tangential momentum, energy */
+    if (sl >= 0.0 || sr <= 0.0) {
+        const double* w = sl >= 0.0 ? wl : wr;
+        double        fe[4];
+        nsc_euler_flux(w, dir, fe);
+        f[0] = fe[0];
+        f[1] = fe[1];
+        f[2] = fe[2];
+        f[3] = fe[3];
+    } else {
+        const int     left = sm >= 0.0;
+        const double* w    = left ? wl : wr;
+        const double  s = left ? sl : sr, un = left ? ul : ur, ut = left ? vl : vr, m = left ? ml : mr;
+        const double  E  = w[3] / (g - 1.0) + 0.5 * w[0] * (un * un + ut * ut);
+        const double  k  = m / (s - sm); /* star density */
+        const double  Es = k * (E / w[0] + (sm - un) * (sm + w[3] / m));
+        double        fe[4];
+        nsc_euler_flux(w, dir, fe);
+        /* F* = F + s (U* - U); U* = (k, k sm, k ut, Es), U = (rho, rho un, rho ut, E) */
+        f[0] = fe[0] + s * (k - w[0]);
+        f[1] = fe[1] + s * (k * sm - w[0] * un);
+        f[2] = fe[2] + s * (k * ut - w[0] * ut);
+        f[3] = fe[3] + s * (Es - E);
+    }
+    F[0] = f[0];
+    F[1] = dir == 0 ? f[1] : f[2];
+    F[2] = dir == 0 ? f[2] : f[1];
+    F[3] = f[3];
+}
+
+/* gl, gr: the cells' central-difference gradients (ux, uy, vx, vy, Tx, Ty); T = p / rho. */
+void or_nsc_visc_f64(const double* wl, const double* wr, const double* gl, const double* gr, int dir, double* F) {
+    const double Tl = wl[3] / wl[0], Tr = wr[3] / wr[0];
+    double       d[6]; /* ux uy vx vy Tx Ty at the face */
+    for (int q = 0; q < 6; ++q) d[q] = 0.5 * (gl[q] + gr[q]);
+    const int n = dir; /* the normal derivatives come from the two cells */
+    d[0 + n]     = wr[1] - wl[1];
+    d[2 + n]     = wr[2] - wl[2];
+    d[4 + n]     = Tr - Tl;
+    const double mu = NSC_MU, kappa = NSC_MU * NSC_GAMMA / ((NSC_GAMMA - 1.0) * NSC_PR);
+    const double dv  = d[0] + d[3];
+    const double txx = mu * (2.0 * d[0] - 2.0 / 3.0 * dv), tyy = mu * (2.0 * d[3] - 2.0 / 3.0 * dv), txy = mu * (d[1] + d[2]);
+    const double uf = 0.5 * (wl[1] + wr[1]), vf = 0.5 * (wl[2] + wr[2]);
+    const double tn1 = dir == 0 ? txx : txy, tn2 = dir == 0 ? txy : tyy;
+    F[0] = 0.0;
+    F[1] = tn1;
+    F[2] = tn2;
+    F[3] = uf * tn1 + vf * tn2 + kappa * d[4 + n];
+}
+
+void or_nsc_cell_diag_f64(const double* W, double* dt, double* ws) {
+    const double c = sqrt(NSC_GAMMA * W[3] / W[0]);
+    const double ax = fabs(W[1]) + c, ay = fabs(W[2]) + c;
+    const double nu = NSC_MU / W[0] * fmax(4.0 / 3.0, NSC_GAMMA / NSC_PR);
+    *dt = NSC_CFL / (ax + ay + 4.0 * nu);
+    *ws = fmax(ax, ay);
+}
+
+void or_nsc_init_f32(int nx, int ny, float amp, float* U) {
+    const float  k  = (float)(2.0 * 3.14159265358979323846 / 32.0);
+    const size_t nc = (size_t)nx * (size_t)ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const float  x = (float)i + 0.5f, y = (float)j + 0.5f;
+            const float  sx = sinf(k * x), cx = cosf(k * x), sy = sinf(k * y), cy = cosf(k * y);
+            const float  r = 1.0f + 0.1f * amp * sx * sy, u = 0.3f + 0.05f * amp * cx * sy, v = 0.2f + 0.05f * amp * sx * cy;
+            const float  p = 1.0f + 0.05f * amp * cx * cy;
+            const size_t c = (size_t)j * (size_t)nx + (size_t)i;
+            U[c]          = r;
+            U[nc + c]     = r * u;
+            U[2 * nc + c] = r * v;
+            U[3 * nc + c] = p / (1.4f - 1.0f) + 0.5f * r * (u * u + v * v);
+        }
+}
+
+/* The padded primitives (rho, u, v, p) of (ny + 4) x (nx + 4) cells, index ((b * PX) + a) * 4 + var. */
+static double* nsc_padded(int nx, int ny, int walls, const float* U) {
+    const long    PX = nx + 4, PY = ny + 4;
+    const size_t  nc = (size_t)nx * (size_t)ny;
+    double* const W  = (double*)malloc(sizeof(double) * 4 * (size_t)PX * (size_t)PY);
+    for (long b = 0; b < PY; ++b)
+        for (long a = 0; a < PX; ++a) {
+            long   j = b - 2, i = ((a - 2) % nx + nx) % nx;
+            double flip = 1.0;
+            if (j < 0 || j >= ny) {
+                if (walls) {
+                    j    = j < 0 ? -1 - j : 2L * ny - 1 - j;
+                    flip = -1.0;
+                } else {
+                    j = (j % ny + ny) % ny;
+                }
+            }
+            const size_t c = (size_t)j * (size_t)nx + (size_t)i;
+            double       u[4] = {U[c], flip * U[nc + c], flip * U[2 * nc + c], U[3 * nc + c]};
+            or_nsc_prim_f64(u, &W[((size_t)b * (size_t)PX + (size_t)a) * 4]);
+        }
+    return W;
+}
+
+/* The convective and viscous fluxes through the face between padded cells (bl, al) and (br, ar). */
+static void nsc_face(const double* W, long PX, long bl, long al, long br, long ar, int dir, int vanleer, double* Fc, double* Fv) {
+#define NSC_W(b, a) (&W[((size_t)(b) * (size_t)PX + (size_t)(a)) * 4])
+    const long db = dir == 1, da = dir == 0;
+    const double *pl = NSC_W(bl, al), *pr = NSC_W(br, ar);
+    const double *pll = NSC_W(bl - db, al - da), *prr = NSC_W(br + db, ar + da);
+    double        wl[4], wr[4], gl[6], gr[6];
+    for (int q = 0; q < 4; ++q) {
+        wl[q] = pl[q] + 0.5 * or_nsc_slope_f64(pl[q] - pll[q], pr[q] - pl[q], vanleer);
+        wr[q] = pr[q] - 0.5 * or_nsc_slope_f64(pr[q] - pl[q], prr[q] - pr[q], vanleer);
+    }
+    or_nsc_hllc_f64(wl, wr, dir, Fc);
+    for (int s = 0; s < 2; ++s) {
+        const long    b = s ? br : bl, a = s ? ar : al;
+        const double *xm = NSC_W(b, a - 1), *xp = NSC_W(b, a + 1), *ym = NSC_W(b - 1, a), *yp = NSC_W(b + 1, a);
+        double* const g = s ? gr : gl;
+        g[0] = 0.5 * (xp[1] - xm[1]);
+        g[1] = 0.5 * (yp[1] - ym[1]);
+        g[2] = 0.5 * (xp[2] - xm[2]);
+        g[3] = 0.5 * (yp[2] - ym[2]);
+        g[4] = 0.5 * (xp[3] / xp[0] - xm[3] / xm[0]);
+        g[5] = 0.5 * (yp[3] / yp[0] - ym[3] / ym[0]);
+    }
+    or_nsc_visc_f64(pl, pr, gl, gr, dir, Fv);
+#undef NSC_W
+}
+
+void or_nsc_residual_f64(int nx, int ny, int walls, int vanleer, const float* U, double* R, double* diag) {
+    const long    PX = nx + 4;
+    double* const W  = nsc_padded(nx, ny, walls, U);
+    const size_t  nc = (size_t)nx * (size_t)ny;
+    double        dtmin = 1e300, wsmax = -1e300, sq[4] = {0, 0, 0, 0};
+#pragma omp parallel for schedule(static) reduction(min : dtmin) reduction(max : wsmax) reduction(+ : sq[:4])
+    for (long j = 0; j < ny; ++j) {
+        const long b = j + 2;
+        for (long i = 0; i < nx; ++i) {
+            const long a = i + 2;
+            double     cw[4], vw[4], ce[4], ve[4], cs[4], vs[4], cn[4], vn[4];
+            nsc_face(W, PX, b, a - 1, b, a, 0, vanleer, cw, vw); /* west */
+            nsc_face(W, PX, b, a, b, a + 1, 0, vanleer, ce, ve); /* east */
+            nsc_face(W, PX, b - 1, a, b, a, 1, vanleer, cs, vs); /* south */
+            nsc_face(W, PX, b, a, b + 1, a, 1, vanleer, cn, vn); /* north */
+            for (int q = 0; q < 4; ++q) {
+                const double divc = (ce[q] - cw[q]) + (cn[q] - cs[q]);
+                const double divv = (ve[q] - vw[q]) + (vn[q] - vs[q]);
+                const double r    = -divc + divv;
+                R[(size_t)q * nc + (size_t)j * (size_t)nx + (size_t)i] = r;
+                sq[q] += r * r;
+            }
+            double dt, ws;
+            or_nsc_cell_diag_f64(&W[((size_t)b * (size_t)PX + (size_t)a) * 4], &dt, &ws);
+            dtmin = fmin(dtmin, dt);
+            wsmax = fmax(wsmax, ws);
+        }
+    }
+    free(W);
+    diag[0] = dtmin;
+    diag[1] = wsmax;
+    for (int q = 0; q < 4; ++q) diag[2 + q] = sq[q];
+}

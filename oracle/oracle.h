@@ -96,6 +96,22 @@ void or_ns_residual_f64(int nx, int ny, double h, double nu, int store, const vo
 void or_ns_compare(int nx, int ny, double h, double nu, int store, const void* u, const void* v, const void* p,
                    const float* Ru, const float* Rv, const float* Rc, double* stats, double* sq);
 
+/* ---- compressible NS residual, FP64: examples/ns-residual/physics.hpp (numerics), reference.hpp:29-128 --------- */
+/* U = (rho, rho u, rho v, E) as [4, ny, nx] f32; walls: no-slip adiabatic walls in y (else periodic); vanleer: the van
+ * Leer limiter (else minmod). Writes R [4, ny, nx] and diag = (min dt, max wave speed, sum R^2 of the 4 variables). */
+void   or_nsc_residual_f64(int nx, int ny, int walls, int vanleer, const float* U, double* R, double* diag);
+/* The pieces, for pinning: conserved -> primitives (rho, u, v, p); the limited slope of (backward a, forward b); HLLC
+ * with Davis speeds through a face with normal x (dir 0) or y (dir 1); the viscous flux from the face's two cells and
+ * their gradients (ux, uy, vx, vy, Tx, Ty); a cell's local timestep and wave speed. */
+void   or_nsc_prim_f64(const double* U, double* W);
+double or_nsc_slope_f64(double a, double b, int vanleer);
+void   or_nsc_hllc_f64(const double* wl, const double* wr, int dir, double* F);
+void   or_nsc_visc_f64(const double* wl, const double* wr, const double* gl, const double* gr, int dir, double* F);
+void   or_nsc_cell_diag_f64(const double* W, double* dt, double* ws);
+/* Uniform flow (0.3, 0.2) at rho = p = 1 plus a perturbation of amplitude amp and wavelength 32 cells, in f32 with the
+ * host's sinf / cosf (the device generator uses its own: tests read the device's field back). */
+void   or_nsc_init_f32(int nx, int ny, float amp, float* U);
+
 /* ---- threads used by the OpenMP loops here ---------------------------------------------------------------------- */
 int or_threads(void);
 
