@@ -94,6 +94,16 @@ WORKLOADS_API = [
     ("cs_chain_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
     ("cs_chain_stats", _I, [_P, C.POINTER(C.c_uint64)]),
     ("cs_chain_destroy", _I, [_P]),
+    ("cs_nsc_create", _I, [C.POINTER(_P), _I, _S, _S, _I, _I, _I]),
+    ("cs_nsc_init", _I, [_P, _F]),
+    ("cs_nsc_write", _I, [_P, C.c_void_p, _S]),
+    ("cs_nsc_run", _I, [_P]),
+    ("cs_nsc_wait", _I, [_P]),
+    ("cs_nsc_read", _I, [_P, C.c_char_p, C.c_void_p, _S]),
+    ("cs_nsc_time", _I, [_P, _I, C.POINTER(_F)]),
+    ("cs_nsc_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
+    ("cs_nsc_stats", _I, [_P, C.POINTER(C.c_uint64)]),
+    ("cs_nsc_destroy", _I, [_P]),
     ("cs_batch_create", _I, [C.POINTER(_P), _U32, _U32, _U32, _F, _F, _U32]),
     ("cs_batch_sizes", _I, [_P, C.POINTER(_U32), _S]),
     ("cs_batch_set_params", _I, [_P, _U32, _U32, _U32]),

@@ -83,6 +83,39 @@ int cs_ns_time(cs_ns m, int iters, float* ms_per_run);
 int cs_ns_time_launches(cs_ns m, int iters, float* ms, int cap, int* count);
 int cs_ns_destroy(cs_ns m);
 
+/* ---- compressible Navier-Stokes residual ----------------------------------------------------------------------------
+ * replaces: the unfused and fused variants of examples/ns-residual (systems.hpp:26-140) and the driver's actions
+ * (main.cpp:159-199, 304-329): R = -div F_conv + div F_visc of U = (rho, rho u, rho v, E) [4, ny, nx] f32 (gamma 1.4, mu 0.01,
+ * Pr 0.72, dx = dy = 1; MUSCL with the minmod or van Leer limiter, HLLC, viscous flux; x periodic, y periodic or no-slip
+ * adiabatic walls) and the diagnostics min local timestep, max wave speed, sum R^2 per variable, atomic and deterministic.
+ *   CS_NSC_UNFUSED   six launches: init, tile, three reductions, error
+ *   CS_NSC_FUSED     one launch with group waits (the reference's layout: init | tile | barrier | reductions | error)
+ *   CS_NSC_LAUNCHES  the same fused group with barriers after init, the tile and the reductions: four launches
+ * nx a multiple of 64, ny of tile_rows (8 or 16: the rows of a work-group's tile). cs_nsc_init sets U from the device
+ * generator (uniform flow plus a perturbation of amplitude amp; 0 gives uniform flow), cs_nsc_write from the host.
+ * Names for cs_nsc_read (f32): "U", "R", "Part" [6, tiles], "Dt" "Ws" (1) "Sq" (4) atomic, "DetDt" "DetWs" "DetSq"
+ * deterministic, "Err" (6: atomic - deterministic for dt, ws, sq0..3).
+ */
+#define CS_NSC_UNFUSED 0
+#define CS_NSC_FUSED 1
+#define CS_NSC_LAUNCHES 2
+#define CS_NSC_MINMOD 0
+#define CS_NSC_VANLEER 1
+
+typedef struct cs_nsc_s* cs_nsc;
+
+int cs_nsc_create(cs_nsc* m, int variant, size_t nx, size_t ny, int limiter, int walls, int tile_rows);
+int cs_nsc_init(cs_nsc m, float amp);
+int cs_nsc_write(cs_nsc m, const void* host_U, size_t bytes);
+int cs_nsc_run(cs_nsc m);
+int cs_nsc_wait(cs_nsc m);
+int cs_nsc_read(cs_nsc m, const char* name, void* host, size_t bytes);
+int cs_nsc_time(cs_nsc m, int iters, float* ms_per_run);
+int cs_nsc_time_launches(cs_nsc m, int iters, float* ms, int cap, int* count);
+/* the model's executor counters: runs, kernel launches, graph captures, graph replays */
+int cs_nsc_stats(cs_nsc m, uint64_t* stats4);
+int cs_nsc_destroy(cs_nsc m);
+
 /* ---- the GEMVER -> BiCGK -> GEMVER chain ------------------------------------------------------------------------------
  * replaces: the Chain System of examples/gemver-chain/systems.hpp:226-346 in its six styles and the driver's actions
  * (main.cpp:154-188). G1: B1 = A + u1 v1^T + u2 v2^T, x1 = beta B1^T ya + za, w1 = alpha B1 x1; BiCGK: q = B1 x1,

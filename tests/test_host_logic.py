@@ -48,3 +48,13 @@ def test_ragged_host_logic():
     build()
     out = run("ragged_host")
     assert "offsets_from" in out and "fused group" in out
+
+
+def test_nsc_host_logic():
+    """The compressible NS residual's three variants: tied atomic accumulators (System::tie_outputs and cs::ties_desc in a
+    fused group) are one variable, the reductions read the tile's partials, shapes resolve, launches per evaluation are
+    6 / 1 / 4, eager and replayed; a plain tie of outputs without allow_plain is refused."""
+    build()
+    out = run("nsc_host")
+    for name, launches in (("unfused", 6), ("fused", 1), ("launches", 4)):
+        assert f"{name}: {launches} launches per evaluation" in out
