@@ -187,6 +187,47 @@ def bench_chain(extra, iters, n=16384, R=64):
     log(f"gemver chain n={n}: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['GBps_nominal']:.0f} GB/s nominal)" for k, v in rec["styles"].items()))
 
 
+# examples/ns-residual/README.md:87-103 (RTX 4090, walls, minmod): ms per evaluation at n = nx = ny
+NSC_REFERENCE_4090_MS = {1024: {"unfused": 3.260, "fused": 1.532, "handwritten": 0.268},
+                         2048: {"unfused": 3.263, "fused": 1.842, "handwritten": 0.945},
+                         4096: {"unfused": 9.685, "fused": 5.810, "handwritten": 3.550}}
+NSC_VARIANTS = ("unfused", "fused", "launches")
+
+
+def bench_nsc(extra, iters):
+    """The compressible NS residual (walls, minmod, as the reference's table): ms per evaluation of each variant and tile
+    height at n = 1024, 2048, 4096, the reference's nominal GB/s (32 B per cell) and the per-launch times of one run."""
+    from cospanmdao_amd import check, workloads as W
+
+    rec = {"config": "walls, minmod, nx = ny = n", "nominal_bytes_per_cell": 32, "reference_rtx4090_ms": NSC_REFERENCE_4090_MS,
+           "runs": {}}
+    for n in (1024, 2048, 4096):
+        res = {}
+        for v, variant in enumerate(NSC_VARIANTS):
+            for ty in (8, 16):
+                h = C.c_void_p()
+                check(W.cs_nsc_create(C.byref(h), v, n, n, 0, 1, ty))
+                check(W.cs_nsc_init(h, 1.0))
+                for _ in range(3):
+                    check(W.cs_nsc_run(h))
+                check(W.cs_nsc_wait(h))
+                t = C.c_float()
+                check(W.cs_nsc_time(h, iters, C.byref(t)))
+                lm = (C.c_float * 8)()
+                cnt = C.c_int()
+                check(W.cs_nsc_time_launches(h, 2, lm, 8, C.byref(cnt)))
+                check(W.cs_nsc_destroy(h))
+                ms = t.value
+                res[f"{variant}-ty{ty}"] = {"ms": round(ms, 4), "GBps_nominal": round(gbs(32 * n * n, ms), 1),
+                                            "launch_ms": [round(lm[i], 4) for i in range(min(cnt.value, 8))]}
+        best = min(res, key=lambda k: res[k]["ms"])
+        ref = NSC_REFERENCE_4090_MS[n]
+        rec["runs"][n] = {"variants": res, "best": best,
+                          "speedup_vs_reference_rtx4090": {k: round(ref[k] / res[best]["ms"], 2) for k in ref}}
+        log(f"nsc n={n}: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in res.items()) + f"; best {best}")
+    extra["ns_compressible"] = rec
+
+
 def bench_precision(extra, steps):
     n = 32768
     rec = {}
@@ -302,6 +343,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched ragged GEMVER")
     ap.add_argument("--ns-only", action="store_true",
                     help="only the NS residual lines (4096^2 and 8192^2 f32, 8192^2 bf16); prints their record, not the bench line")
+    ap.add_argument("--nsc-only", action="store_true",
+                    help="only the compressible NS residual at n = 1024, 2048, 4096 in three variants; prints its record, not the bench line")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
     ap.add_argument("--batch-only", action="store_true",
@@ -320,6 +363,11 @@ def main():
         extra = {}
         bench_ns(extra, max(20, a.steps // 4))
         print(json.dumps({k: extra[k] for k in ("ns4096_f32", "ns8192_f32", "ns8192_bf16")}), flush=True)
+        return
+    if a.nsc_only:
+        extra = {}
+        bench_nsc(extra, max(20, a.steps // 4))
+        print(json.dumps(extra["ns_compressible"]), flush=True)
         return
     if a.chain_only:
         extra = {}
@@ -373,6 +421,7 @@ def main():
             bench_batch(extra, value, max(30, a.steps // 2))
         bench_precision(extra, max(20, a.steps // 8))
         bench_chain(extra, max(10, a.steps // 10))
+        bench_nsc(extra, max(10, a.steps // 10))
 
     cpu = None
     if not (a.no_cpu or a.quick):

@@ -17,7 +17,7 @@
 //
 // NscTile<Lim, Walls, TY> (the MI355X design of the tile pipeline, one stage): a work-group of 256 owns a tile of TY rows
 // x 64 columns. It loads the tile with a 2-cell halo once (4 coalesced loads per cell, boundary rows mirrored or wrapped
-// on the way) and converts it to primitives and temperature in LDS; computes EVERY face of its cells (the x-faces of the
+// on the way) and converts it to primitives in LDS; computes EVERY face of its cells (the x-faces of the
 // tile's east edge and the y-faces of its north edge too, which the neighbouring tile computes again: 1/64 and 1/TY more
 // face work instead of any exchange between work-groups), x-faces first into an LDS flux buffer, their divergence into
 // registers, then the y-faces into the same buffer; and finishes each cell: R to HBM, the cell's timestep and wave speed,
@@ -122,7 +122,9 @@ __device__ inline void viscous(float dnu, float dnv, float dnT, float tu, float 
 
 }  // namespace detail
 
-/// The LDS of a tile: 5 fields (rho u v p T) of (TY + 4) x 68 cells, then 4 flux fields (one direction at a time).
+/// The LDS of a tile: 4 fields (rho u v p) of (TY + 4) x 68 cells, then 4 flux fields (one direction at a time). The
+/// temperatures of a face's two cells are recomputed (two reciprocals) rather than stored: at 16 rows that keeps a tile
+/// under 40 KB, four work-groups per CU instead of three (the tile is VALU-bound and needs the waves to hide latency).
 using TileMem = cs::LocalMem<cs::LocalArray<DType::f32, 1>>;
 
 // =============================================================================
@@ -135,10 +137,10 @@ public:
     static constexpr int PW = kTX + 2 * kHalo, PH = TY + 2 * kHalo, PC = PW * PH;
     static constexpr int NFX = TY * (kTX + 1), NFY = (TY + 1) * kTX;
     static constexpr int FS          = NFX > NFY ? NFX : NFY;  ///< one flux field
-    static constexpr int kLocalElems = 5 * PC + 4 * FS;
+    static constexpr int kLocalElems = 4 * PC + 4 * FS;
     static constexpr int kCells      = TY * kTX / kWg;  ///< cells per work-item
     static_assert(TY * kTX % kWg == 0, "a tile is a whole number of cells per work-item");
-    static_assert(kLocalElems * 4 <= 64 * 1024, "a tile's LDS fits three work-groups per CU");
+    static_assert(kLocalElems * 4 <= 40 * 1024, "a tile's LDS lets four work-groups share a CU");
 
     struct Inputs {
         cs::Input<DType::f32, 3> U;  ///< [4, ny, nx] conserved
@@ -152,8 +154,8 @@ public:
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out, TileMem lm) {
         using namespace detail;
         auto [lt]          = lm.get(it);
-        float* const P     = &lt[std::size_t{0}];  // P + f * PC: rho, u, v, p, T
-        float* const Fl    = P + 5 * PC;            // Fl + q * FS: the face fluxes of variable q
+        float* const P     = &lt[std::size_t{0}];  // P + f * PC: rho, u, v, p
+        float* const Fl    = P + 4 * PC;            // Fl + q * FS: the face fluxes of variable q
         const int    tid   = static_cast<int>(it.get_local_id(0));
         const auto   shape = in.U.shape();
         const int    ny = static_cast<int>(shape[1]), nx = static_cast<int>(shape[2]);
@@ -164,7 +166,7 @@ public:
         const std::size_t plane = static_cast<std::size_t>(nx) * static_cast<std::size_t>(ny);
         const float*      U     = cs::raw::data(in.U);
 
-        // ---- the tile and its halo: conserved -> primitives and temperature ----
+        // ---- the tile and its halo: conserved -> primitives ----
         for (int e = tid; e < PC; e += kWg) {
             const int b = e / PW, a = e - b * PW;
             int       gx = x0 - kHalo + a;
@@ -189,12 +191,11 @@ public:
             P[PC + e]           = u;
             P[2 * PC + e]       = v;
             P[3 * PC + e]       = p;
-            P[4 * PC + e]       = p * ir;
         }
         __syncthreads();
         const float* const Pu = P + PC;
         const float* const Pv = P + 2 * PC;
-        const float* const PT = P + 4 * PC;
+        const float* const Pp = P + 3 * PC;
 
         // ---- x-faces: face c of tile row r lies between cells (r, c - 1) and (r, c) ----
         for (int f = tid; f < NFX; f += kWg) {
@@ -211,7 +212,7 @@ public:
             hllc<0>(wl, wr, Fc);
             const float tu = 0.5f * (0.5f * (Pu[e + PW] - Pu[e - PW]) + 0.5f * (Pu[e + 1 + PW] - Pu[e + 1 - PW]));
             const float tv = 0.5f * (0.5f * (Pv[e + PW] - Pv[e - PW]) + 0.5f * (Pv[e + 1 + PW] - Pv[e + 1 - PW]));
-            viscous<0>(Pu[e + 1] - Pu[e], Pv[e + 1] - Pv[e], PT[e + 1] - PT[e], tu, tv, 0.5f * (Pu[e] + Pu[e + 1]), 0.5f * (Pv[e] + Pv[e + 1]), Fv);
+            viscous<0>(Pu[e + 1] - Pu[e], Pv[e + 1] - Pv[e], Pp[e + 1] * rcp(P[e + 1]) - Pp[e] * rcp(P[e]), tu, tv, 0.5f * (Pu[e] + Pu[e + 1]), 0.5f * (Pv[e] + Pv[e + 1]), Fv);
 #pragma unroll
             for (int q = 0; q < 4; ++q) Fl[q * FS + f] = Fc[q] - Fv[q];
         }
@@ -240,7 +241,7 @@ public:
             hllc<1>(wl, wr, Fc);
             const float tu = 0.5f * (0.5f * (Pu[e + 1] - Pu[e - 1]) + 0.5f * (Pu[e + PW + 1] - Pu[e + PW - 1]));
             const float tv = 0.5f * (0.5f * (Pv[e + 1] - Pv[e - 1]) + 0.5f * (Pv[e + PW + 1] - Pv[e + PW - 1]));
-            viscous<1>(Pu[e + PW] - Pu[e], Pv[e + PW] - Pv[e], PT[e + PW] - PT[e], tu, tv, 0.5f * (Pu[e] + Pu[e + PW]),
+            viscous<1>(Pu[e + PW] - Pu[e], Pv[e + PW] - Pv[e], Pp[e + PW] * rcp(P[e + PW]) - Pp[e] * rcp(P[e]), tu, tv, 0.5f * (Pu[e] + Pu[e + PW]),
                        0.5f * (Pv[e] + Pv[e + PW]), Fv);
 #pragma unroll
             for (int q = 0; q < 4; ++q) Fl[q * FS + f] = Fc[q] - Fv[q];
@@ -263,7 +264,7 @@ public:
             }
             const int   e  = (r + kHalo) * PW + c + kHalo;
             const float ir = rcp(P[e]);
-            const float cs_ = root(kGamma * P[3 * PC + e] * ir);
+            const float cs_ = root(kGamma * Pp[e] * ir);
             const float ax = fabsf(Pu[e]) + cs_, ay = fabsf(Pv[e]) + cs_;
             dt             = fminf(dt, kCfl * rcp(ax + ay + 4.0f * (kMu * ir * kNuMax)));
             ws             = fmaxf(ws, fmaxf(ax, ay));
@@ -282,13 +283,19 @@ public:
             for (int q = 0; q < 4; ++q) part[(2 + q) * n + t] = sq[q];
         }
         // the atomics after the init stage (in a fused group with waits it is group 0 of segment 0; elsewhere it ran in an
-        // earlier launch and the wait is a no-op)
+        // earlier launch and the wait is a no-op). Device-scope atomics on one address serialize across the XCDs: a min
+        // (max) is only sent when the tile's value is below (above) the accumulator's current value, which only decreases
+        // (increases) after the reset, so the result is the same; the four sums go out from four work-items at once.
         it.wait_group(0, cs::id<1>{0});
         if (tid == 0) {
-            out.Dt[cs::id<1>{0}].fetch_min(dt);
-            out.Ws[cs::id<1>{0}].fetch_max(ws);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) out.Sq[cs::id<1>{static_cast<std::size_t>(q)}].fetch_add(sq[q]);
+            const auto a = out.Dt[cs::id<1>{0}];
+            if (dt < a.load()) a.fetch_min(dt);
+        } else if (tid == 1) {
+            const auto a = out.Ws[cs::id<1>{0}];
+            if (ws > a.load()) a.fetch_max(ws);
+        } else if (tid < 6) {
+            const int q = tid - 2;
+            out.Sq[cs::id<1>{static_cast<std::size_t>(q)}].fetch_add(q == 0 ? sq[0] : q == 1 ? sq[1] : q == 2 ? sq[2] : sq[3]);
         }
     }
 
