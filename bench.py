@@ -194,14 +194,14 @@ NSC_REFERENCE_4090_MS = {1024: {"unfused": 3.260, "fused": 1.532, "handwritten":
 NSC_VARIANTS = ("unfused", "fused", "launches")
 
 
-def bench_nsc(extra, iters):
+def bench_nsc(extra, iters, sizes=(1024, 2048, 4096)):
     """The compressible NS residual (walls, minmod, as the reference's table): ms per evaluation of each variant and tile
     height at n = 1024, 2048, 4096, the reference's nominal GB/s (32 B per cell) and the per-launch times of one run."""
     from cospanmdao_amd import check, workloads as W
 
     rec = {"config": "walls, minmod, nx = ny = n", "nominal_bytes_per_cell": 32, "reference_rtx4090_ms": NSC_REFERENCE_4090_MS,
            "runs": {}}
-    for n in (1024, 2048, 4096):
+    for n in sizes:
         res = {}
         for v, variant in enumerate(NSC_VARIANTS):
             for ty in (8, 16):
@@ -221,7 +221,7 @@ def bench_nsc(extra, iters):
                 res[f"{variant}-ty{ty}"] = {"ms": round(ms, 4), "GBps_nominal": round(gbs(32 * n * n, ms), 1),
                                             "launch_ms": [round(lm[i], 4) for i in range(min(cnt.value, 8))]}
         best = min(res, key=lambda k: res[k]["ms"])
-        ref = NSC_REFERENCE_4090_MS[n]
+        ref = NSC_REFERENCE_4090_MS.get(n, {})
         rec["runs"][n] = {"variants": res, "best": best,
                           "speedup_vs_reference_rtx4090": {k: round(ref[k] / res[best]["ms"], 2) for k in ref}}
         log(f"nsc n={n}: " + ", ".join(f"{k} {v['ms']:.3f} ms" for k, v in res.items()) + f"; best {best}")
@@ -345,6 +345,7 @@ def main():
                     help="only the NS residual lines (4096^2 and 8192^2 f32, 8192^2 bf16); prints their record, not the bench line")
     ap.add_argument("--nsc-only", action="store_true",
                     help="only the compressible NS residual at n = 1024, 2048, 4096 in three variants; prints its record, not the bench line")
+    ap.add_argument("--nsc-sizes", default="1024,2048,4096", help="grid sizes n (nx = ny) of --nsc-only, comma separated")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
     ap.add_argument("--batch-only", action="store_true",
@@ -366,7 +367,7 @@ def main():
         return
     if a.nsc_only:
         extra = {}
-        bench_nsc(extra, max(20, a.steps // 4))
+        bench_nsc(extra, max(20, a.steps // 4), tuple(int(v) for v in a.nsc_sizes.split(",")))
         print(json.dumps(extra["ns_compressible"]), flush=True)
         return
     if a.chain_only:

@@ -198,7 +198,8 @@ inline constexpr std::size_t kTicketHeader = 2;
 CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return kTicketHeader + segments + total; }
 
 /// A spin-wait load: relaxed at agent scope (coherent across the XCDs' L2s without invalidating them on every poll); the
-/// waits below order what follows with ONE acquire fence once the condition holds.
+/// waits below order what follows with ONE acquire fence once the condition holds, made by the leader before the
+/// work-group barrier (the barrier carries it to the other waves: one cache invalidation per work-group, not per wave).
 __device__ inline std::uint32_t poll_load(std::uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 /// Waits only ever target groups with earlier tickets, which have started and finish without waiting on later ones, so
@@ -212,9 +213,9 @@ __device__ inline void ticket_wait_group(std::uint32_t* ts, std::uint32_t segmen
         const std::uint64_t t = tbl[s].first + segment_linear(tbl[s], c);
         for (std::uint32_t spin = 0; poll_load(&ts[kTicketHeader + segments + t]) == 0 && spin < kSpinLimit; ++spin)
             __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 /// COLLECTIVE: wait until every group of segments 0 .. upto-1 has finished (reference kernel.hpp:227-236).
@@ -224,9 +225,9 @@ __device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo
             for (std::uint32_t spin = 0;
                  static_cast<std::uint64_t>(poll_load(&ts[kTicketHeader + i])) != tbl[i].count && spin < kSpinLimit; ++spin)
                 __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 __device__ inline std::uint32_t local_linear_id() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
@@ -837,15 +838,19 @@ __global__ __launch_bounds__(MaxThreads) void group_kernel(const GroupArgs<X...>
     run_stages<N>(ga, lg, cs_group_lds, std::make_index_sequence<N>{});
 
     if (ts != nullptr) {
-        // completion: barrier, release, the group's flag and its segment's count; the last group zeroes the state
+        // completion: barrier, ONE release fence (it orders the group's writes before the flag and the counts), the group's
+        // flag and its segment's count, then the finished count with release (the flag and the count before it); only the
+        // last group acquires, and zeroes the state. An acquire per group would invalidate its XCD's L2 every time.
         __syncthreads();
         __shared__ std::uint32_t s_last;
         if (local_linear_id() == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&ts[kTicketHeader + geo.segments + ticket], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&ts[kTicketHeader + lg.segment], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            const std::uint32_t f = __hip_atomic_fetch_add(&ts[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            s_last                = (static_cast<std::uint64_t>(f) + 1 == geo.total_groups) ? 1u : 0u;
+            __hip_atomic_store(&ts[kTicketHeader + geo.segments + ticket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&ts[kTicketHeader + lg.segment], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const std::uint32_t f    = __hip_atomic_fetch_add(&ts[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            const bool          last = static_cast<std::uint64_t>(f) + 1 == geo.total_groups;
+            if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_last = last ? 1u : 0u;
         }
         __syncthreads();
         if (s_last) {
