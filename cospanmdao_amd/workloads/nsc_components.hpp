@@ -358,8 +358,8 @@ public:
 };
 
 // =============================================================================
-// The deterministic reductions of the per-tile partials (fixed order: each work-item a fixed stride of tiles, then the
-// group's fixed tree); norms in FP64
+// The deterministic reductions of the per-tile partials (fixed order: each work-item a fixed stride of tiles, 8 loads in
+// flight, folded in tile order, then the group's fixed tree); norms in FP64
 // =============================================================================
 /// Kind 0: the four sums of squares (four groups, group q sums row 2 + q); 1: the min dt (row 0); 2: the max wave speed.
 template <int Kind>
@@ -376,20 +376,34 @@ public:
         const std::size_t  n = in.Part.shape()[1], tid = it.get_local_id(0), q = it.get_group(0);
         const float* const part = cs::raw::data(in.Part);
         const auto         grp  = it.get_group();
+        // each work-item: tiles tid, tid + 256, ... taken 8 at a time (8 loads in flight), folded in that order
+        constexpr int kIn = 8;
+        const float* const row = part + (Kind == 0 ? (2 + q) * n : Kind == 1 ? 0 : n);
         if constexpr (Kind == 0) {
             double s = 0.0;
-            for (std::size_t t = tid; t < n; t += kWg) s += static_cast<double>(part[(2 + q) * n + t]);
+            for (std::size_t t0 = tid; t0 < n; t0 += kWg * kIn) {
+                float v[kIn];
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) v[k] = t0 + k * kWg < n ? row[t0 + k * kWg] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) s += static_cast<double>(v[k]);
+            }
             s = cs::reduce_over_group(grp, s, cs::plus<double>());
             if (tid == 0) out.Det[cs::id<1>{q}] = static_cast<float>(s);
-        } else if constexpr (Kind == 1) {
-            float m = std::numeric_limits<float>::infinity();
-            for (std::size_t t = tid; t < n; t += kWg) m = fminf(m, part[t]);
-            m = cs::reduce_over_group(grp, m, cs::minimum<float>());
-            if (tid == 0) out.Det[cs::id<1>{0}] = m;
         } else {
-            float m = -std::numeric_limits<float>::infinity();
-            for (std::size_t t = tid; t < n; t += kWg) m = fmaxf(m, part[n + t]);
-            m = cs::reduce_over_group(grp, m, cs::maximum<float>());
+            constexpr bool lo = Kind == 1;
+            float          m  = lo ? std::numeric_limits<float>::infinity() : -std::numeric_limits<float>::infinity();
+            for (std::size_t t0 = tid; t0 < n; t0 += kWg * kIn) {
+                float v[kIn];
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) v[k] = t0 + k * kWg < n ? row[t0 + k * kWg] : m;
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) m = lo ? fminf(m, v[k]) : fmaxf(m, v[k]);
+            }
+            if constexpr (lo)
+                m = cs::reduce_over_group(grp, m, cs::minimum<float>());
+            else
+                m = cs::reduce_over_group(grp, m, cs::maximum<float>());
             if (tid == 0) out.Det[cs::id<1>{0}] = m;
         }
     }
