@@ -48,6 +48,8 @@ public:
 
     void sync() {
         detail::VarMeta& v = *var_;
+        if (!v.data && v.shape_known() && v.elements() > 0)
+            throw std::logic_error("naga::cospan: view: '" + v.name + "' has no data: every subsystem holding it is out of scope");
         if (v.write) detail::hip_check(cs_hip_event_synchronize(v.write->get()), "view sync");
         for (int d = 0; d < Rank; ++d) shape_[d] = v.dims[d];
         data_.resize(v.elements());
@@ -168,23 +170,28 @@ public:
         detail::ModelMeta& m = *meta_;
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
+        const std::vector<bool> active = begin_turn(m);
+        ++stats_.runs;
         if (const auto steps = ragged_steps(m); !steps.empty()) {
-            run_stepwise(m, steps);
-            ++stats_.runs;
+            run_stepwise(m, steps, active);
+            end_turn(m, active);
             return;
         }
-        std::vector<detail::LaunchRecord> recs;
-        recs.reserve(m.leaves.size());
-        std::vector<std::byte> sig;
-        for (detail::Leaf* leaf : m.leaves) {
-            recs.push_back(leaf->prepare(m));
-            recs.back().append_signature(sig);
+        std::vector<detail::LaunchRecord> recs(m.leaves.size());
+        std::vector<std::byte>            sig;
+        for (std::size_t k = 0; k < m.leaves.size(); ++k) {
+            if (!active[k]) continue;
+            recs[k] = m.leaves[k]->prepare(m);
+            const std::uint64_t tag = k;
+            const auto*         b   = reinterpret_cast<const std::byte*>(&tag);
+            sig.insert(sig.end(), b, b + sizeof tag);
+            recs[k].append_signature(sig);
         }
-        ++stats_.runs;
         detail::Executor& ex = m.exec;
         if (replay_enabled_ && ex.has_graph() && sig == graph_sig_) {
             ex.replay();
-            after_graph(m);
+            after_graph(m, active);
+            end_turn(m, active);
             return;
         }
         if (replay_enabled_ && !sig.empty() && sig == last_sig_) {
@@ -194,19 +201,52 @@ public:
                 inst->var->write.reset();
                 inst->var->reads.clear();
             }
-            for (std::size_t k = 0; k < recs.size(); ++k) ex.submit(*m.leaves[k], recs[k]);
+            submit_all(m, recs, active);
             ex.end_capture();
             graph_sig_ = sig;
             ex.replay();
-            after_graph(m);
+            after_graph(m, active);
+            end_turn(m, active);
             return;
         }
         if (ex.has_graph()) {
             ex.drop_graph();
             graph_sig_.clear();
         }
-        for (std::size_t k = 0; k < recs.size(); ++k) ex.submit(*m.leaves[k], recs[k]);
+        submit_all(m, recs, active);
         last_sig_ = std::move(sig);
+        end_turn(m, active);
+    }
+
+    /// Puts the subsystem at `path` (and everything under it) back in scope and resets their policies' counters
+    /// (reference model.hpp:544-552). Legal between runs.
+    void rearm_subsystem(std::string_view path) {
+        detail::ModelMeta& m = *meta_;
+        if (!m.has_node(path)) throw std::invalid_argument("naga::cospan: Model::rearm_subsystem: no subsystem '" + std::string{path} + "'");
+        for (auto& [p, st] : m.node_state)
+            if (detail::ModelMeta::under(p, path)) {
+                st.in_scope = true;
+                if (st.policy) st.policy->rearm();
+            }
+    }
+    /// Whether the subsystem at `path` is in scope (it and every subsystem above it).
+    bool in_scope(std::string_view path) const {
+        const detail::ModelMeta& m = *meta_;
+        if (!m.has_node(path)) throw std::invalid_argument("naga::cospan: Model::in_scope: no subsystem '" + std::string{path} + "'");
+        for (const auto& [p, st] : m.node_state)
+            if (!st.in_scope && detail::ModelMeta::under(path, p)) return false;
+        return true;
+    }
+    /// The run policy of the subsystem at `path` (absolute), as System::set_run_policy.
+    void set_run_policy(std::string_view path, const property_list& policy = {}) {
+        detail::ModelMeta& m = *meta_;
+        if (!m.has_node(path)) throw std::invalid_argument("naga::cospan: set_run_policy: no subsystem '" + std::string{path} + "'");
+        m.node_state[std::string{path}].policy = detail::make_run_policy(policy, "set_run_policy('" + std::string{path} + "')");
+    }
+    void clear_run_policy(std::string_view path) {
+        detail::ModelMeta& m = *meta_;
+        if (!m.has_node(path)) throw std::invalid_argument("naga::cospan: clear_run_policy: no subsystem '" + std::string{path} + "'");
+        m.node_state[std::string{path}].policy.reset();
     }
 
     /// Waits for every launch of the model.
@@ -218,22 +258,25 @@ public:
         detail::ModelMeta& m = *meta_;
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
-        const auto steps = ragged_steps(m);
-        std::vector<detail::LaunchRecord> recs;
+        const std::vector<bool>           active = begin_turn(m);
+        const auto                        steps  = ragged_steps(m);
+        std::vector<detail::LaunchRecord> recs(m.leaves.size());
         if (steps.empty())
-            for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
+            for (std::size_t k = 0; k < m.leaves.size(); ++k)
+                if (active[k]) recs[k] = m.leaves[k]->prepare(m);
         std::vector<detail::Executor::TimedLaunch> t;
         m.exec.set_timing(&t);
         try {
             if (!steps.empty())
-                run_stepwise(m, steps);
+                run_stepwise(m, steps, active);
             else
-                for (std::size_t k = 0; k < recs.size(); ++k) m.exec.submit(*m.leaves[k], recs[k]);
+                submit_all(m, recs, active);
         } catch (...) {
             m.exec.set_timing(nullptr);
             throw;
         }
         m.exec.set_timing(nullptr);
+        end_turn(m, active);
         m.exec.synchronize();
         std::vector<std::pair<std::string, float>> out;
         for (auto& tl : t) {
@@ -256,6 +299,8 @@ public:
         m.exec.init();
         for (auto& h : m.hooks) h.apply(m, *h.inst->var);
         if (!ragged_steps(m).empty()) throw std::invalid_argument("naga::cospan: time_leaf: a model with ragged shape hooks runs leaf by leaf; time it with run_timed");
+        for (const auto& [p, st] : m.node_state)
+            if (st.policy || !st.in_scope) throw std::invalid_argument("naga::cospan: time_leaf: a model with run policies or subsystems out of scope; time it with run_timed");
         std::vector<detail::LaunchRecord> recs;
         for (detail::Leaf* leaf : m.leaves) recs.push_back(leaf->prepare(m));
         if (k >= recs.size())
@@ -387,19 +432,115 @@ private:
     }
 
     /// One eager forward leaf by leaf with the ragged hooks as steps between the leaves.
-    void run_stepwise(detail::ModelMeta& m, const RaggedSteps& steps) {
+    void run_stepwise(detail::ModelMeta& m, const RaggedSteps& steps, const std::vector<bool>& active) {
         detail::Executor& ex = m.exec;
         if (ex.has_graph()) {
             ex.drop_graph();
             graph_sig_.clear();
         }
         last_sig_.clear();
-        std::size_t si = 0;
+        std::size_t                        si = 0;
+        const auto                         deps = depends_of(m);
+        std::vector<detail::EventPtr>      ev(m.leaves.size());
         for (std::size_t k = 0;; ++k) {
             while (si < steps.size() && steps[si].first == k) ragged_step(m, *steps[si++].second);
             if (k == m.leaves.size()) break;
+            if (!active[k]) continue;
             const detail::LaunchRecord rec = m.leaves[k]->prepare(m);
-            ex.submit(*m.leaves[k], rec);
+            ev[k]                          = ex.submit(*m.leaves[k], rec, extra_deps(deps[k], ev));
+        }
+    }
+
+    /// For every leaf, the leaves it waits on through depends_on.
+    static std::vector<std::vector<std::size_t>> depends_of(const detail::ModelMeta& m) {
+        std::vector<std::vector<std::size_t>> d(m.leaves.size());
+        for (const auto& [b, a] : m.depends) {
+            const auto la = m.leaves_under(a);
+            for (std::size_t k : m.leaves_under(b)) d[k].insert(d[k].end(), la.begin(), la.end());
+        }
+        return d;
+    }
+    static std::vector<detail::EventPtr> extra_deps(const std::vector<std::size_t>& on, const std::vector<detail::EventPtr>& ev) {
+        std::vector<detail::EventPtr> out;
+        for (std::size_t j : on)
+            if (ev[j]) out.push_back(ev[j]);
+        return out;
+    }
+    /// Launches the prepared records of the active leaves in order, with their depends_on waits.
+    static void submit_all(detail::ModelMeta& m, const std::vector<detail::LaunchRecord>& recs, const std::vector<bool>& active) {
+        const auto                    deps = depends_of(m);
+        std::vector<detail::EventPtr> ev(m.leaves.size());
+        for (std::size_t k = 0; k < recs.size(); ++k)
+            if (active[k]) ev[k] = m.exec.submit(*m.leaves[k], recs[k], extra_deps(deps[k], ev));
+    }
+
+    /// The start of a turn (reference run_policy.hpp, TransientPolicy::before): a transient subsystem out of scope whose
+    /// parents are in scope is rearmed for this turn. Returns which leaves run.
+    std::vector<bool> begin_turn(detail::ModelMeta& m) {
+        for (auto& [path, st] : m.node_state)
+            if (st.policy && st.policy->kind == detail::RunPolicy::Kind::transient && !st.in_scope) {
+                bool parents = true;
+                for (const auto& [p, o] : m.node_state)
+                    if (p != path && !o.in_scope && detail::ModelMeta::under(path, p)) parents = false;
+                if (parents) rearm_subsystem(path);
+            }
+        std::vector<bool> active(m.leaves.size());
+        for (std::size_t k = 0; k < m.leaves.size(); ++k) active[k] = m.leaf_in_scope(*m.leaves[k]);
+        return active;
+    }
+
+    /// The end of a turn (CountedPolicy / TransientPolicy::after): a counted subsystem that ran counts a turn and is
+    /// dearmed after its last; a transient one is dearmed. Dearming waits for the device, then releases the memory of
+    /// every variable that no leaf in scope holds (a free input with an initializer is initialized again when needed).
+    void end_turn(detail::ModelMeta& m, const std::vector<bool>& active) {
+        std::vector<std::string> dearm;
+        for (auto& [path, st] : m.node_state) {
+            if (!st.policy) continue;
+            bool ran = false;
+            for (std::size_t k : m.leaves_under(path)) ran = ran || active[k];
+            bool visited = true;  // in scope this turn (it and its parents)
+            for (const auto& [p, o] : m.node_state)
+                if (!o.in_scope && detail::ModelMeta::under(path, p)) visited = false;
+            if (!visited && !ran) continue;
+            if (st.policy->kind == detail::RunPolicy::Kind::transient) {
+                dearm.push_back(path);
+            } else if (st.policy->left > 0 && --st.policy->left == 0) {
+                dearm.push_back(path);
+            }
+        }
+        if (dearm.empty()) return;
+        for (const std::string& path : dearm) {
+            for (const std::string& n : m.nodes)
+                if (detail::ModelMeta::under(n, path)) m.node_state[n].in_scope = false;
+        }
+        m.exec.synchronize();
+        std::vector<std::pair<detail::VarMeta*, detail::IOKind>> pv;
+        std::vector<detail::VarMeta*>                            held;  // held by a leaf in scope
+        for (detail::Leaf* leaf : m.leaves)
+            if (m.leaf_in_scope(*leaf)) {
+                pv.clear();
+                leaf->port_vars(pv);
+                for (const auto& [v, kind] : pv) held.push_back(v);
+            }
+        bool released = false;
+        for (detail::Leaf* leaf : m.leaves) {
+            if (m.leaf_in_scope(*leaf)) continue;
+            pv.clear();
+            leaf->port_vars(pv);
+            for (const auto& [v, kind] : pv) {
+                if (!v->data || std::find(held.begin(), held.end(), v) != held.end()) continue;
+                v->release();
+                v->has_value = false;
+                v->write.reset();
+                v->reads.clear();
+                if (v->init && !v->has_producer) v->pending = true;
+                released = true;
+            }
+        }
+        if (released && m.exec.has_graph()) {
+            m.exec.drop_graph();
+            graph_sig_.clear();
+            last_sig_.clear();
         }
     }
 
@@ -460,21 +601,24 @@ private:
     /// After a graph launch on stream 0: every variable's last event is the graph's completion.
     /// After a replay on stream 0: the graph's tail (a deferred event) is every written variable's writer and every read
     /// variable's only reader (it subsumes earlier replays' reads, which ran before it on the same stream).
-    void after_graph(detail::ModelMeta& m) {
+    void after_graph(detail::ModelMeta& m, const std::vector<bool>& active) {
         const detail::EventPtr e = m.exec.deferred(0);
-        for (detail::Leaf* leaf : m.leaves) {
-            for (detail::VarMeta* v : leaf->writes) {
+        for (std::size_t k = 0; k < m.leaves.size(); ++k) {
+            if (!active[k]) continue;
+            for (detail::VarMeta* v : m.leaves[k]->writes) {
                 v->write = e;
                 v->reads.clear();
                 v->has_value = true;
             }
         }
-        for (detail::Leaf* leaf : m.leaves)
-            for (detail::VarMeta* v : leaf->reads)
+        for (std::size_t k = 0; k < m.leaves.size(); ++k) {
+            if (!active[k]) continue;
+            for (detail::VarMeta* v : m.leaves[k]->reads)
                 if (v->write != e) {
                     v->reads.clear();
                     v->reads.push_back(e);
                 }
+        }
     }
 
     std::unique_ptr<detail::ModelMeta> meta_;

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <optional>
 #include <stdexcept>
 #include <string>
 #include <string_view>
@@ -30,6 +31,7 @@
 
 #include "cospan_hip.h"
 #include "naga/cospan/group_launch.hpp"
+#include "naga/cospan/run_policy.hpp"
 #include "naga/cospan/variable.hpp"
 
 namespace naga::cospan::detail {
@@ -130,10 +132,12 @@ public:
     /// While set, every submit records timing events around its launch into `sink`.
     void set_timing(std::vector<TimedLaunch>* sink) { timing_ = sink; }
 
-    /// Launches one prepared leaf with its dependencies and records its event on every variable it touched.
-    void submit(Leaf& leaf, const LaunchRecord& rec) {
-        if (rec.empty()) return;
-        const std::vector<EventPtr> deps = deps_of(leaf.reads, leaf.writes);
+    /// Launches one prepared leaf with its dependencies (and `extra` ones: depends_on) and records its event on every
+    /// variable it touched; returns that event (null for an empty record).
+    EventPtr submit(Leaf& leaf, const LaunchRecord& rec, const std::vector<EventPtr>& extra = {}) {
+        if (rec.empty()) return nullptr;
+        std::vector<EventPtr> deps = deps_of(leaf.reads, leaf.writes);
+        deps.insert(deps.end(), extra.begin(), extra.end());
         const int                   s    = choose(deps);
         join_capture(s);
         for (const EventPtr& d : deps) wait(s, d);
@@ -162,6 +166,7 @@ public:
             v->reads.clear();
             v->has_value = true;
         }
+        return ev;
     }
 
     /// An initializer's work on stream 0: it overwrites the variable, so it waits like a writer.
@@ -264,10 +269,19 @@ struct HookRecord {
     std::shared_ptr<DeviceColumn>             column;  ///< reused over forwards
 };
 
+/// A subsystem's run state: its policy and whether it is in scope.
+struct NodeState {
+    std::optional<RunPolicy> policy;
+    bool                     in_scope = true;
+};
+
 struct ModelMeta {
     std::map<std::string, std::shared_ptr<Instance>, std::less<>> instances;
     std::vector<Leaf*>                                            leaves;
     std::vector<HookRecord>                                       hooks;
+    std::vector<std::string>                                      nodes;  ///< every subsystem path, in order of addition
+    std::map<std::string, NodeState, std::less<>>                 node_state;
+    std::vector<std::pair<std::string, std::string>>              depends;  ///< (b, a): b's leaves wait on a's
     Executor                                                      exec;
     bool                                                          in_setup  = false;
     bool                                                          finalized = false;
@@ -275,6 +289,28 @@ struct ModelMeta {
     static std::string join(std::string_view base, std::string_view rel) {
         if (base.empty()) return std::string{rel};
         return std::string{base} + "." + std::string{rel};
+    }
+    /// `path` is `node` or lies under it.
+    static bool under(std::string_view path, std::string_view node) {
+        return path == node || (path.size() > node.size() && path.substr(0, node.size()) == node && path[node.size()] == '.');
+    }
+    bool has_node(std::string_view path) const {
+        for (const std::string& n : nodes)
+            if (n == path) return true;
+        return false;
+    }
+    /// The indices of the leaves under `node` (empty when it has none).
+    std::vector<std::size_t> leaves_under(std::string_view node) const {
+        std::vector<std::size_t> out;
+        for (std::size_t k = 0; k < leaves.size(); ++k)
+            if (under(leaves[k]->path, node)) out.push_back(k);
+        return out;
+    }
+    /// A leaf runs when neither it nor any subsystem above it is out of scope.
+    bool leaf_in_scope(const Leaf& leaf) const {
+        for (const auto& [path, st] : node_state)
+            if (!st.in_scope && under(leaf.path, path)) return false;
+        return true;
     }
 
     std::shared_ptr<Instance> find(std::string_view full) const {

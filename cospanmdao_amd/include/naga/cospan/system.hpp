@@ -181,10 +181,21 @@ public:
 
     const std::string& path() const { return path_; }
 
+    /// A subsystem with a run policy (run_n, do_once or transient; reference system.hpp:125-128).
+    template <class T, class... A>
+    void add_subsystem(const property_list& props, SubsystemDesc<T, A...> desc) {
+        const std::string full = detail::ModelMeta::join(path_, desc.name);
+        auto              pol  = detail::make_run_policy(props, "add_subsystem('" + full + "')");
+        add_subsystem(std::move(desc));
+        if (pol) meta("add_subsystem").node_state[full].policy = pol;
+    }
+
     template <class T, class... A>
     void add_subsystem(SubsystemDesc<T, A...> desc) {
         detail::ModelMeta& m    = meta("add_subsystem");
         const std::string  full = detail::ModelMeta::join(path_, desc.name);
+        if (m.has_node(full)) throw std::invalid_argument("naga::cospan: add_subsystem: '" + full + "' already exists");
+        m.nodes.push_back(full);
         if constexpr (std::is_base_of_v<CustomComponent<T>, T>) {
             std::unique_ptr<T> c = detail::make_subsystem<T>(desc.args);
             detail::ComponentAccess::bind(*c, full, &m);
@@ -302,6 +313,40 @@ public:
     /// counts, totals past 2^32 - 1 are errors), as the reference's (shape_hook.hpp:425-473).
     void shape_hook(std::string_view target, ragged::CountsFrom c) { ragged_hook(target, c.path, false); }
     void shape_hook(std::string_view target, ragged::OffsetsFrom o) { ragged_hook(target, o.path, true); }
+
+    /// `b` depends on `a` (subsystems of this System added in that order; reference system.hpp:233-250): in every run, each
+    /// leaf of b waits for every leaf of a that ran, with or without a variable between them. Kept in graph replay (the
+    /// waits are captured with the launches). A repeated edge is ignored.
+    void depends_on(std::string_view b, std::string_view a) {
+        detail::ModelMeta& m  = meta("depends_on");
+        const std::string  fb = detail::ModelMeta::join(path_, b), fa = detail::ModelMeta::join(path_, a);
+        if (!m.has_node(fb)) throw std::invalid_argument("naga::cospan: depends_on: '" + std::string{b} + "' is not a subsystem of '" + path_ + "'");
+        if (!m.has_node(fa)) throw std::invalid_argument("naga::cospan: depends_on: '" + std::string{a} + "' is not a subsystem of '" + path_ + "'");
+        const auto la = m.leaves_under(fa), lb = m.leaves_under(fb);
+        if (!la.empty() && !lb.empty() && la.back() >= lb.front())
+            throw std::invalid_argument("naga::cospan: depends_on: '" + std::string{b} + "' can depend only on a subsystem added before it ('" +
+                                        std::string{a} + "' was not)");
+        for (const auto& e : m.depends)
+            if (e.first == fb && e.second == fa) return;
+        m.depends.emplace_back(fb, fa);
+    }
+
+    /// Sets the run policy of the subsystem at `rel` (relative to this System; a leaf or a System): {} none, {run_n{n}},
+    /// {do_once} or {transient}. The last call wins and its counter starts over; the scope is not changed
+    /// (Model::rearm_subsystem puts a subsystem back in scope).
+    void set_run_policy(std::string_view rel, const property_list& policy = {}) {
+        detail::ModelMeta& m    = meta("set_run_policy");
+        const std::string  full = detail::ModelMeta::join(path_, rel);
+        if (!m.has_node(full)) throw std::invalid_argument("naga::cospan: set_run_policy: no subsystem '" + full + "'");
+        m.node_state[full].policy = detail::make_run_policy(policy, "set_run_policy('" + full + "')");
+    }
+    /// Removes the run policy of the subsystem at `rel`: it just runs.
+    void clear_run_policy(std::string_view rel) {
+        detail::ModelMeta& m    = meta("clear_run_policy");
+        const std::string  full = detail::ModelMeta::join(path_, rel);
+        if (!m.has_node(full)) throw std::invalid_argument("naga::cospan: clear_run_policy: no subsystem '" + full + "'");
+        m.node_state[full].policy.reset();
+    }
 
     /// The initial value of a free input (a child's input or an alias of this System).
     void init_input(std::string_view target, InitSpec init) {

@@ -98,19 +98,22 @@ void enqueue(cs_hip_stream s, std::function<void()> fn) {
 
 bool runnable(const Op& op) { return op.kind != Op::wait || op.ev->completed >= op.gen; }
 
-/// Runs queued work until every queue is empty: the highest-numbered stream with runnable work first.
+/// Runs queued work until every queue is empty, one operation at a time, always from the highest-numbered stream that has
+/// runnable work (an operation that unblocks a higher stream hands over to it at once).
 void drain() {
     State& st = S();
     for (;;) {
         bool progress = false, pending = false;
         for (auto it = st.streams.rbegin(); it != st.streams.rend(); ++it) {
             Stream& q = **it;
-            while (!q.q.empty() && runnable(q.q.front())) {
+            if (!q.q.empty() && runnable(q.q.front())) {
                 Op op = std::move(q.q.front());
                 q.q.pop_front();
                 if (op.kind == Op::work) op.fn();
                 if (op.kind == Op::record) op.ev->completed = std::max(op.ev->completed, op.gen);
                 progress = true;
+                pending  = true;
+                break;
             }
             pending = pending || !q.q.empty();
         }

@@ -29,3 +29,14 @@ def test_ragged_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+
+
+def test_policies_program():
+    """Run policies, rearm and depends_on through cs::Model on the device: tests/framework/policies.cpp (the reference's
+    test_r04.cpp policy and dearm cases and test_system.cpp's depends_on side channel, over captured and replayed runs)."""
+    exe = os.path.join(HERE, "framework", "_build", "policies")
+    assert os.path.exists(exe), "tests/framework/_build/policies is not built (make -C tests/framework)"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]

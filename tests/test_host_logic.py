@@ -58,3 +58,12 @@ def test_nsc_host_logic():
     out = run("nsc_host")
     for name, launches in (("unfused", 6), ("fused", 1), ("launches", 4)):
         assert f"{name}: {launches} launches per evaluation" in out
+
+
+def test_policies_host_logic():
+    """Run policies (run_n, do_once, transient), rearm, released memory and refused views, and depends_on through a side
+    channel (tests/framework/policies.cpp against the mock boundary): without the edge the mock's stream order makes the
+    reader run first and read the stale value; with it, every eager, captured and replayed run reads the writer's value."""
+    build()
+    out = run("policies_host")
+    assert "depends_on: B reads what A wrote" in out and "transient" in out
