@@ -40,3 +40,15 @@ def test_policies_program():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+
+
+def test_ragged_gemver_program():
+    """Batched GEMVERs of different sizes written only with ragged ports and counts_from hooks (tests/framework/
+    ragged_gemver.cpp): bare and fused passes, sizes and instance counts changing between forwards, every instance against
+    FP64 (rel-L2 <= 1e-5)."""
+    exe = os.path.join(HERE, "framework", "_build", "ragged_gemver")
+    assert os.path.exists(exe), "tests/framework/_build/ragged_gemver is not built (make -C tests/framework)"
+    print("running", exe, flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
