@@ -345,6 +345,8 @@ def main():
                     help="only the NS residual lines (4096^2 and 8192^2 f32, 8192^2 bf16); prints their record, not the bench line")
     ap.add_argument("--nsc-only", action="store_true",
                     help="only the compressible NS residual at n = 1024, 2048, 4096 in three variants; prints its record, not the bench line")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only GEMVER N=32768 with A stored f32, bf16 and e4m3; prints its record, not the bench line")
     ap.add_argument("--nsc-sizes", default="1024,2048,4096", help="grid sizes n (nx = ny) of --nsc-only, comma separated")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
@@ -364,6 +366,11 @@ def main():
         extra = {}
         bench_ns(extra, max(20, a.steps // 4))
         print(json.dumps({k: extra[k] for k in ("ns4096_f32", "ns8192_f32", "ns8192_bf16")}), flush=True)
+        return
+    if a.precision_only:
+        extra = {}
+        bench_precision(extra, max(20, a.steps // 8))
+        print(json.dumps(extra["gemver32768_storage"]), flush=True)
         return
     if a.nsc_only:
         extra = {}

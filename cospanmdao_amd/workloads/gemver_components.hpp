@@ -159,22 +159,27 @@ public:
     }
 };
 
-template <bool Z>
-class ColReduce : public cs::Component<ColReduce<Z>> {
+/// Dots: also the partial dot products of x with v1 and v2 over each work-group's columns, dots[0][g] and dots[1][g]
+/// (a fixed-order group reduction), for the factored pass 2 of reduced-precision storage.
+template <bool Z, bool Dots = false>
+class ColReduce : public cs::Component<ColReduce<Z, Dots>> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct InputsZ {
         cs::Input<DType::f32, 2> P;
         cs::Input<DType::f32, 1> z;
         cs::Input<DType::f32, 1> scale;
+        cs::Input<DType::f32, 1> v1, v2;  ///< Dots only
     };
     struct InputsNoZ {
         cs::Input<DType::f32, 2> P;
         cs::Input<DType::f32, 1> scale;
+        cs::Input<DType::f32, 1> v1, v2;  ///< Dots only
     };
     using Inputs = std::conditional_t<Z, InputsZ, InputsNoZ>;
     struct Outputs {
         cs::Output<DType::f32, 1> x;
+        cs::Output<DType::f32, 2> dots;  ///< Dots only: [2, work-groups]
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         const std::uint32_t n = u32(in.P.shape()[1]), strips = u32(in.P.shape()[0]);
@@ -182,6 +187,34 @@ public:
         const std::uint32_t stride = u32(it.get_group_range(0) * it.get_local_range(0));
         const float*        P      = cs::raw::data(in.P);
         const std::size_t   ld     = in.P.stride()[0];
+        if constexpr (Dots) {
+            // one column per work-item (the launch covers n exactly once)
+            const std::uint32_t j = u32(it.get_global_id(0));
+            float               p1 = 0.0f, p2 = 0.0f;
+            if (j < n) {
+                float        acc = 0.0f;
+                const float* pj  = P + j;
+#pragma unroll 16
+                for (std::uint32_t c = 0; c < strips; ++c) acc += pj[static_cast<std::size_t>(c) * ld];
+                float xj;
+                if constexpr (Z)
+                    xj = fmaf(scale, acc, in.z[cs::id<1>{j}]);
+                else
+                    xj = scale * acc;
+                out.x[cs::id<1>{j}] = xj;
+                p1                  = in.v1[cs::id<1>{j}] * xj;
+                p2                  = in.v2[cs::id<1>{j}] * xj;
+            }
+            const auto grp = it.get_group();
+            p1             = cs::reduce_over_group(grp, p1, cs::plus<float>());
+            p2             = cs::reduce_over_group(grp, p2, cs::plus<float>());
+            if (it.get_local_id(0) == 0) {
+                const std::size_t g = it.get_group(0);
+                out.dots[cs::id<2>{0, g}] = p1;
+                out.dots[cs::id<2>{1, g}] = p2;
+            }
+            return;
+        }
         for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
             float        acc = 0.0f;
             const float* pj  = P + j;
@@ -203,8 +236,17 @@ public:
         if constexpr (Z) this->declare_input(inputs.z, "z", cs::unspecified_shape<1>());
         this->declare_input(inputs.scale, "scale", cs::unspecified_shape<1>());
         this->declare_output(outputs.x, "x", cs::unspecified_shape<1>());
+        if constexpr (Dots) {
+            this->declare_input(inputs.v1, "v1", cs::unspecified_shape<1>());
+            this->declare_input(inputs.v2, "v2", cs::unspecified_shape<1>());
+            this->declare_output(outputs.dots, "dots", cs::unspecified_shape<2>());
+        }
     }
-    void prepare_forward() override { this->set_shape(outputs.x, this->shape_of(inputs.P)[1]); }
+    void prepare_forward() override {
+        const std::size_t n = this->shape_of(inputs.P)[1];
+        this->set_shape(outputs.x, n);
+        if constexpr (Dots) this->set_shape(outputs.dots, 2, round_up(n, kWg) / kWg);
+    }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
         const std::size_t range = inputs.P.shape()[1];
         return this->create_launch_config(cs::range_desc(cs::range<1>{round_up(range, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
@@ -360,6 +402,13 @@ public:
     static constexpr int kChunk = kWg * kEl;
     /// columns of one work-group
     static constexpr int kCols = kChunk * kV;
+    /// e4m3 storage sums the stored values against s_i y_i and adds the rank-2 terms once per column:
+    /// P[c][j] = sum_i q_ij (s_i y_i) + v1_j sum_i u1_i y_i + v2_j sum_i u2_i y_i (one fma per element instead of three and
+    /// a scale). f32 keeps the components' order (bit-exact); bf16 keeps it too (its passes are bound by HBM either way,
+    /// and the factored form measured slower for it).
+    static constexpr bool kFactored = S == DType::e4m3;
+    /// rows in flight per work-item: 16 loads of 16 bytes (f32 4 rows x 4 chunks, bf16 8 x 2, e4m3 16 x 1)
+    static constexpr int kRowsInFlight = 16 / kV;
 
     struct Inputs {
         cs::Input<S, 2>          A;
@@ -377,9 +426,16 @@ public:
     __device__ static void compute_forward(cs::GroupItem<2> it, const Inputs& in, const Outputs& out) {
         const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
         const std::size_t   lda  = in.A.stride()[0];
-        const std::uint32_t R    = in.R[std::size_t{0}];
+        // R in a scalar register: the strip's row indices are then uniform and the per-row values (y, u1, u2, the row
+        // scale) load through the scalar cache instead of as vector loads of one address (four per row next to one 16-byte
+        // load of A for e4m3)
+        const std::uint32_t R = static_cast<std::uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(in.R[std::size_t{0}])));
         const std::uint32_t c = u32(it.get_group(0)), cb = u32(it.get_group(1));
         const std::uint32_t jb = cb * kCols + u32(it.get_local_id(1)) * kEl;
+        if constexpr (kFactored) {
+            factored(it, in, out, R, c, jb);
+            return;
+        }
         if (jb >= n) return;
         const std::uint32_t i0 = c * R, i1 = (i0 + R < rows) ? i0 + R : rows;
         const auto*         A  = cs::raw::data(in.A);
@@ -452,6 +508,101 @@ public:
         }
     }
 
+    /// Reduced-precision storage: the strip's s_i y_i staged in LDS by the work-group (one coalesced load of each per-row
+    /// value instead of a load of one address per work-item and row), sum u1_i y_i and sum u2_i y_i reduced over the group
+    /// in a fixed order, then P[c][j] = sum_i q_ij (s_i y_i) + v1_j sum u1_i y_i + v2_j sum u2_i y_i. Every work-item stays
+    /// to the end (the barriers), those past the row only skip the loads and stores.
+    __device__ static void factored(cs::GroupItem<2> it, const Inputs& in, const Outputs& out, std::uint32_t R, std::uint32_t c,
+                                    std::uint32_t jb) {
+        const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
+        const std::size_t   lda  = in.A.stride()[0];
+        const std::uint32_t i0 = c * R, i1 = (i0 + R < rows) ? i0 + R : rows;
+        const std::uint32_t tid = u32(it.get_local_id(1));
+        const auto*         A   = cs::raw::data(in.A);
+        const float *       y = cs::raw::data(in.y), *u1 = cs::raw::data(in.u1), *u2 = cs::raw::data(in.u2);
+        const float *       sc = cs::raw::data(in.row_scale), *V1 = cs::raw::data(in.v1), *V2 = cs::raw::data(in.v2);
+        __shared__ float    ys[kWg];
+
+        bool          full[kV];
+        std::uint32_t part_q = kV, part_m = 0;
+        float         acc[kV][kEl];
+#pragma unroll
+        for (int q = 0; q < kV; ++q) {
+            const std::uint32_t j = jb + static_cast<std::uint32_t>(q) * kChunk;
+            full[q]               = j + kEl <= n;
+            if (!full[q] && j < n) {
+                part_q = static_cast<std::uint32_t>(q);
+                part_m = n - j;
+            }
+#pragma unroll
+            for (int e = 0; e < kEl; ++e) acc[q][e] = 0.0f;
+        }
+        float pacc[kEl];
+#pragma unroll
+        for (int e = 0; e < kEl; ++e) pacc[e] = 0.0f;
+        float t1 = 0.0f, t2 = 0.0f;  // this work-item's rows of the strip: u1_i y_i, u2_i y_i
+        for (std::uint32_t b0 = i0; b0 < i1; b0 += kWg) {
+            const std::uint32_t nb = i1 - b0 < static_cast<std::uint32_t>(kWg) ? i1 - b0 : static_cast<std::uint32_t>(kWg);
+            if (tid < nb) {
+                const std::uint32_t i  = b0 + tid;
+                const float         yi = y[i];
+                ys[tid]                = (S == DType::e4m3 ? sc[i] : 1.0f) * yi;
+                t1                     = fmaf(u1[i], yi, t1);
+                t2                     = fmaf(u2[i], yi, t2);
+            }
+            __syncthreads();
+            const typename St::T* a = A + static_cast<std::size_t>(b0) * lda + jb;
+#pragma unroll kRowsInFlight
+            for (std::uint32_t k = 0; k < nb; ++k, a += lda) {
+                typename St::Raw r[kV];
+#pragma unroll
+                for (int q = 0; q < kV; ++q)
+                    if (full[q]) r[q] = load_stream<typename St::Raw>(a + q * kChunk);
+                const float yk = ys[k];
+#pragma unroll
+                for (int q = 0; q < kV; ++q) {
+                    if (!full[q]) continue;
+                    float av[kEl];
+                    St::widen(r[q], 1.0f, av);
+#pragma unroll
+                    for (int e = 0; e < kEl; ++e) acc[q][e] = fmaf(av[e], yk, acc[q][e]);
+                }
+            }
+            if (part_q < kV) {  // the ragged end of the row: element by element
+                const std::uint32_t j = jb + part_q * kChunk;
+                for (std::uint32_t k = 0; k < nb; ++k)
+                    for (std::uint32_t e = 0; e < part_m; ++e)
+                        pacc[e] = fmaf(St::widen1(A[static_cast<std::size_t>(b0 + k) * lda + j + e], 1.0f), ys[k], pacc[e]);
+            }
+            __syncthreads();
+        }
+        t1 = cs::reduce_over_group(it.get_group(), t1, cs::plus<float>());
+        t2 = cs::reduce_over_group(it.get_group(), t2, cs::plus<float>());
+        if (jb >= n) return;
+        float* p = &out.P[cs::id<2>{c, 0}];
+#pragma unroll
+        for (int q = 0; q < kV; ++q) {
+            if (!full[q]) continue;
+            const std::uint32_t j = jb + static_cast<std::uint32_t>(q) * kChunk;
+            float               v1[kEl], v2[kEl];
+            load_f32<kEl>(V1 + j, v1);
+            load_f32<kEl>(V2 + j, v2);
+            using F4  = float __attribute__((ext_vector_type(4)));
+            float* pq = p + j;
+#pragma unroll
+            for (int k4 = 0; k4 < kEl / 4; ++k4) {
+                F4 o;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) o[e] = fmaf(v2[4 * k4 + e], t2, fmaf(v1[4 * k4 + e], t1, acc[q][4 * k4 + e]));
+                reinterpret_cast<F4*>(pq)[k4] = o;
+            }
+        }
+        if (part_q < kV) {
+            const std::uint32_t j = jb + part_q * kChunk;
+            for (std::uint32_t e = 0; e < part_m; ++e) p[j + e] = fmaf(V2[j + e], t2, fmaf(V1[j + e], t1, pacc[e]));
+        }
+    }
+
     Inputs  inputs;
     Outputs outputs;
 
@@ -495,12 +646,19 @@ public:
     static constexpr int kVec                = St::kVec;
     static constexpr int kRows               = RT;
     static constexpr bool kCompensated       = S != DType::f32;
+    /// e4m3 storage: w_i = alpha (s_i sum_j q_ij x_j + u1_i (v1 . x) + u2_i (v2 . x)) with the two dots from the column
+    /// reduction (one fma per stored element instead of three and a scale, and no v1, v2 re-reads per work-group: at one
+    /// byte per element those re-reads outweighed A)
+    static constexpr bool kFactored = S == DType::e4m3;
+    /// column chunks in flight per work-item for e4m3: 16 loads of 16 bytes of A (RT rows each); f32 and bf16 as tuned
+    static constexpr int kChunksInFlight = (kFactored && RT < 16) ? 16 / RT : 1;
 
     struct Inputs {
         cs::Input<S, 2>          A;
         cs::Input<DType::f32, 1> row_scale;
         cs::Input<DType::f32, 1> u1, v1, u2, v2, x;
         cs::Input<DType::f32, 1> scale;
+        cs::Input<DType::f32, 2> dots;  ///< factored only: the column reduction's partial v1 . x, v2 . x
     };
     struct Outputs {
         cs::Output<DType::f32, 1> w;
@@ -530,18 +688,54 @@ public:
             comp[r]               = 0.0f;
         }
 
+        // Neumaier: acc + comp is the running sum of the partials
+        const auto add_comp = [](float& acc_, float& comp_, float part) {
+            const float t = acc_ + part;
+            comp_ += fabsf(acc_) >= fabsf(part) ? (acc_ - t) + part : (part - t) + acc_;
+            acc_ = t;
+        };
+        // factored: v1 . x and v2 . x, the column reduction's partials summed in a fixed order (each work-item a stride of
+        // them, then the group's tree): the chunk loop then reads only x next to A
+        float D1 = 0.0f, D2 = 0.0f;
+        if constexpr (kFactored) {
+            const std::uint32_t g = u32(in.dots.shape()[1]);
+            const float*        dp = cs::raw::data(in.dots);
+            float               p1 = 0.0f, p2 = 0.0f;
+            for (std::uint32_t k = tid; k < g; k += kWg) {
+                p1 += dp[k];
+                p2 += dp[g + k];
+            }
+            D1 = cs::reduce_over_group(it.get_group(), p1, cs::plus<float>());
+            D2 = cs::reduce_over_group(it.get_group(), p2, cs::plus<float>());
+        }
         const std::uint32_t step = kWg * kVec;
         std::uint32_t       j0   = tid * kVec;
+        // chunks unrolled so that 16 loads of 16 bytes of A are in flight per work-item
+#pragma unroll kChunksInFlight
         for (; j0 + kVec <= n; j0 += step) {
             float v1[kVec], v2[kVec], x[kVec];
-            load_f32<kVec>(v1p + j0, v1);
-            load_f32<kVec>(v2p + j0, v2);
+            if constexpr (!kFactored) {
+                load_f32<kVec>(v1p + j0, v1);
+                load_f32<kVec>(v2p + j0, v2);
+            }
             load_f32<kVec>(xv + j0, x);
             typename St::Raw raw[RT];
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
                 const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
                 raw[r]                 = load_stream<typename St::Raw>(A + static_cast<std::size_t>(r0 + rr) * lda + j0);
+            }
+            if constexpr (kFactored) {
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    float av[kVec];
+                    St::widen(raw[r], 1.0f, av);
+                    float part = 0.0f;
+#pragma unroll
+                    for (int v = 0; v < kVec; ++v) part = fmaf(av[v], x[v], part);
+                    add_comp(acc[r], comp[r], part);
+                }
+                continue;
             }
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
@@ -561,7 +755,14 @@ public:
                 }
             }
         }
-        if (j0 < n) {  // the ragged tail of the rows (this work-item's last, partial chunk)
+        if (j0 < n && kFactored) {  // the ragged tail of the rows (this work-item's last, partial chunk)
+            for (int r = 0; r < RT; ++r) {
+                const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                float               part = 0.0f;
+                for (std::uint32_t j = j0; j < n; ++j) part = fmaf(St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], 1.0f), xv[j], part);
+                add_comp(acc[r], comp[r], part);
+            }
+        } else if (j0 < n) {  // the ragged tail of the rows (this work-item's last, partial chunk)
             for (int r = 0; r < RT; ++r) {
                 const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
                 float               part = 0.0f;
@@ -580,7 +781,8 @@ public:
         }
 
         // per row: wave butterfly, then the pairwise sum of the four wave results
-        __shared__ float wave_sums[kWg / cs::wave::kSize][RT];
+        constexpr int    kSums = RT;
+        __shared__ float wave_sums[kWg / cs::wave::kSize][kSums];
         const int        wv = static_cast<int>(tid) / cs::wave::kSize;
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
@@ -591,8 +793,15 @@ public:
         __syncthreads();
         if (tid < nr) {
             const float alpha = in.scale[std::size_t{0}];
-            const float sum   = (wave_sums[0][tid] + wave_sums[1][tid]) + (wave_sums[2][tid] + wave_sums[3][tid]);
-            out.w[cs::id<1>{r0 + tid}] = alpha * sum;
+            const auto  total = [&](int k) { return (wave_sums[0][k] + wave_sums[1][k]) + (wave_sums[2][k] + wave_sums[3][k]); };
+            const float sum   = total(static_cast<int>(tid));
+            if constexpr (kFactored) {
+                const std::uint32_t i  = r0 + tid;
+                const float         si = S == DType::e4m3 ? sc[i] : 1.0f;
+                out.w[cs::id<1>{i}]    = alpha * fmaf(in.u2[cs::id<1>{i}], D2, fmaf(in.u1[cs::id<1>{i}], D1, si * sum));
+            } else {
+                out.w[cs::id<1>{r0 + tid}] = alpha * sum;
+            }
         }
     }
 
@@ -608,6 +817,7 @@ public:
         this->declare_input(inputs.v2, "v2", cs::unspecified_shape<1>());
         this->declare_input(inputs.x, "x", cs::unspecified_shape<1>());
         this->declare_input(inputs.scale, "scale", cs::unspecified_shape<1>());
+        if constexpr (kFactored) this->declare_input(inputs.dots, "dots", cs::unspecified_shape<2>());
         this->declare_output(outputs.w, "w", cs::unspecified_shape<1>());
     }
     void prepare_forward() override { this->set_shape(outputs.w, this->shape_of(inputs.A)[0]); }

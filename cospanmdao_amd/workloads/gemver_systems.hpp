@@ -92,26 +92,41 @@ public:
         const std::size_t n = static_cast<std::size_t>(n_), strips = (n + R_ - 1) / R_;
         std::string       pre;
         if (style_ == Style::twopass || style_ == Style::twopass_1) {
+            // reduced-precision storage: the column reduction also gives v1 . x and v2 . x to the factored pass 2
+            constexpr bool dots = AhatGemv<S, RT>::kFactored;
+            using Cred          = ColReduce<true, dots>;
             if (style_ == Style::twopass_1) {
-                add_subsystem(cs::fuse_components("fw", cs::connects_desc({{"p1.P", "cred.P"}, {"cred.x", "p2.x"}}),
-                                                  cs::subsystem_desc<AhatTPartial<S>>("p1", R_), cs::global_barrier(),
-                                                  cs::subsystem_desc<ColReduce<true>>("cred"), cs::global_barrier(),
-                                                  cs::subsystem_desc<AhatGemv<S, RT>>("p2")));
+                if constexpr (dots)
+                    add_subsystem(cs::fuse_components("fw", cs::connects_desc({{"p1.P", "cred.P"}, {"cred.x", "p2.x"}, {"cred.dots", "p2.dots"}}),
+                                                      cs::subsystem_desc<AhatTPartial<S>>("p1", R_), cs::global_barrier(),
+                                                      cs::subsystem_desc<Cred>("cred"), cs::global_barrier(),
+                                                      cs::subsystem_desc<AhatGemv<S, RT>>("p2")));
+                else
+                    add_subsystem(cs::fuse_components("fw", cs::connects_desc({{"p1.P", "cred.P"}, {"cred.x", "p2.x"}}),
+                                                      cs::subsystem_desc<AhatTPartial<S>>("p1", R_), cs::global_barrier(),
+                                                      cs::subsystem_desc<Cred>("cred"), cs::global_barrier(),
+                                                      cs::subsystem_desc<AhatGemv<S, RT>>("p2")));
                 pre = "fw.";
             } else {
                 add_subsystem(cs::subsystem_desc<AhatTPartial<S>>("p1", R_));
-                add_subsystem(cs::subsystem_desc<ColReduce<true>>("cred"));
+                add_subsystem(cs::subsystem_desc<Cred>("cred"));
                 add_subsystem(cs::subsystem_desc<AhatGemv<S, RT>>("p2"));
                 connect("p1.P", "cred.P");
                 connect("cred.x", "p2.x");
+                if constexpr (dots) connect("cred.dots", "p2.dots");
             }
             // both passes read the same A and rank-2 vectors: one variable each
             tie_inputs<S, 2>({pre + "p1.A", pre + "p2.A"}, "A");
             tie_inputs<DType::f32, 1>({pre + "p1.row_scale", pre + "p2.row_scale"}, "row_scale");
             tie_inputs<DType::f32, 1>({pre + "p1.u1", pre + "p2.u1"}, "u1");
-            tie_inputs<DType::f32, 1>({pre + "p1.v1", pre + "p2.v1"}, "v1");
+            if constexpr (dots) {
+                tie_inputs<DType::f32, 1>({pre + "p1.v1", pre + "p2.v1", pre + "cred.v1"}, "v1");
+                tie_inputs<DType::f32, 1>({pre + "p1.v2", pre + "p2.v2", pre + "cred.v2"}, "v2");
+            } else {
+                tie_inputs<DType::f32, 1>({pre + "p1.v1", pre + "p2.v1"}, "v1");
+                tie_inputs<DType::f32, 1>({pre + "p1.v2", pre + "p2.v2"}, "v2");
+            }
             tie_inputs<DType::f32, 1>({pre + "p1.u2", pre + "p2.u2"}, "u2");
-            tie_inputs<DType::f32, 1>({pre + "p1.v2", pre + "p2.v2"}, "v2");
             promote<DType::f32, 1>(pre + "p1.y", "y");
             promote<DType::f32, 1>(pre + "cred.z", "z");
             promote<DType::f32, 1>(pre + "cred.scale", "beta");
