@@ -16,7 +16,7 @@
 // the fast one, because a waiting protocol needs every group to make its writes visible at agent scope (an L2
 // write-back per group) where a launch boundary does it once.
 // A stage outside a segment is a segment of its own. The group's stage types are a compile-time list (the item types),
-// so its kernel is detail::group_kernel<MaxThreads, X...>, instantiated by hipcc in the translation unit that builds the
+// so its kernel is detail::group_kernel<MaxThreads, MinWaves, X...>, instantiated by hipcc in the translation unit that builds the
 // model: the compile-time counterpart of the reference's JIT-assembled indexed call sequence (kernel.hpp:911-1004).
 
 #include <memory>

@@ -156,12 +156,13 @@ public:
     static_assert(kGroupRange || kIndexRange,
                   "cospan: the stages of one fused group are all group-range or all index-range components");
     static constexpr int kMaxThreads = std::min({max_wg_of<X>::value...});
+    static constexpr int kMinWaves   = std::max({min_waves_of<X>::value...});
 
     /// The host stub of this group's kernel (what the process-wide kernel cache of the reference keys by stage sequence:
     /// here the C++ type system does it, one instantiation per sequence).
     static const void* kernel() {
         if constexpr (kGroupRange)
-            return reinterpret_cast<const void*>(&group_kernel<kMaxThreads, X...>);
+            return reinterpret_cast<const void*>(&group_kernel<kMaxThreads, kMinWaves, X...>);
         else
             return reinterpret_cast<const void*>(&index_kernel<X...>);
     }

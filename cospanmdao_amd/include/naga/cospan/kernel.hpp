@@ -719,6 +719,14 @@ template <class X, class = void>
 struct max_wg_of : std::integral_constant<int, 1024> {};
 template <class X>
 struct max_wg_of<X, std::void_t<decltype(X::max_work_group_size)>> : std::integral_constant<int, X::max_work_group_size> {};
+
+/// The fewest waves per SIMD a stage asks the fused kernel to keep resident (X::min_waves_per_eu, else 1): the kernel's
+/// register budget is then at most 512 / waves VGPRs, so a stage that keeps many loads in flight per wave does not trade
+/// them for occupancy.
+template <class X, class = void>
+struct min_waves_of : std::integral_constant<int, 1> {};
+template <class X>
+struct min_waves_of<X, std::void_t<decltype(X::min_waves_per_eu)>> : std::integral_constant<int, X::min_waves_per_eu> {};
 }  // namespace detail
 
 template <class X>
@@ -813,8 +821,8 @@ __device__ inline void run_stages(const GroupArgs<X...>& ga, const LogicalGroup&
 }
 
 /// THE fused kernel of a group-range group: one instantiation per distinct stage sequence (reference kernel.hpp:1015-1067).
-template <int MaxThreads, class... X>
-__global__ __launch_bounds__(MaxThreads) void group_kernel(const GroupArgs<X...> ga) {
+template <int MaxThreads, int MinWaves, class... X>
+__global__ __launch_bounds__(MaxThreads) __attribute__((amdgpu_waves_per_eu(MinWaves))) void group_kernel(const GroupArgs<X...> ga) {
     extern __shared__ __attribute__((aligned(16))) std::byte cs_group_lds[];
     constexpr int        N   = sizeof...(X);
     const Geometry<N>&   geo = ga.geo;

@@ -18,9 +18,9 @@
 // intermediates in global memory (components.hpp:636-852).
 //
 // NsTile<S>: a work-group owns TY rows x (256 * V) columns; each work-item owns V = 4 consecutive columns (one 16-byte
-// load per field and row for f32 fields, 8 bytes for bf16) and marches down its rows with a sliding window of rows in
-// registers (i - 1, i, i + 1, and i + 2 loading while row i is computed), so every field element is read from HBM once
-// (plus two halo rows per tile) and a row's loads have a whole iteration to arrive. The x neighbours of a work-item's V columns come from the adjacent
+// load per field and row for f32 fields, 8 bytes for bf16) and marches down its rows with a ring of five rows in
+// registers (i - 1, i, i + 1 in use, i + 2 in flight, i + 3 loading while row i is computed), so every field element is
+// read from HBM once (plus two halo rows per tile) and a row's loads have two iterations to arrive. The x neighbours of a work-item's V columns come from the adjacent
 // lanes by DPP wave shifts; only lanes 0 and 63 load their halo column from memory. Tiles are mapped to XCDs in
 // contiguous runs, so the halo rows of consecutive tiles meet in the same L2. S is the storage of u, v, p: f32 or bf16.
 
@@ -113,6 +113,7 @@ template <DType S>
 class NsTile : public cs::Component<NsTile<S>> {
 public:
     static constexpr int max_work_group_size = kWg;
+    static constexpr int min_waves_per_eu    = 4;  ///< the five-row ring fits 128 VGPRs: four work-groups per CU
     using F                                  = Field<S>;
     static constexpr int kVec                = F::kVec;
     static constexpr int kCols               = kWg * kVec;
@@ -144,64 +145,75 @@ public:
         const auto*         V  = cs::raw::data(in.v);
         const auto*         P  = cs::raw::data(in.p);
         const std::size_t   ld = in.u.stride()[0];
-        // which neighbours come from the adjacent lane (else from memory, with the periodic wrap)
-        const bool left_lane  = lane > 0 && j0 > 0 && live;
-        const bool right_lane = lane < 63 && j0 + kVec < nx && live;
-        const std::uint32_t jl = j0 == 0 ? nx - 1 : j0 - 1, jr = j0 + kVec >= nx ? 0 : j0 + kVec;
+        // The x neighbours a lane cannot take from the adjacent lane: lane 0's left neighbour and the right neighbour of
+        // the wave's last live lane (with the periodic wrap). Both columns are the same for the whole wave.
+        const std::uint32_t jw0  = j0r - lane * kVec < nx ? j0r - lane * kVec : nx - kVec;  // the wave's first column
+        const std::uint32_t last = (nx - kVec - jw0) / kVec < 63 ? (nx - kVec - jw0) / kVec : 63;
+        const std::uint32_t jwl  = jw0 + last * kVec;  // the last live lane's first column
+        const std::uint32_t jl = jw0 == 0 ? nx - 1 : jw0 - 1, jr = jwl + kVec >= nx ? 0 : jwl + kVec;
+        const bool left_mem = lane == 0, right_mem = lane == last;
 
-        // A row of the window: the work-item's kVec columns of u, v, p and, for the lanes whose x neighbour is not in the
-        // adjacent lane (the wave's edge lanes, the periodic wrap), that neighbour from memory.
+        // A row of the ring: the work-item's kVec columns of u, v, p as loaded (widened to f32 where they are used).
         struct Row {
-            float u[kVec], v[kVec], p[kVec];
+            typename F::Raw u, v, p;
+        };
+        // The wave's two halo columns of u, v, p (wave-uniform addresses, loaded by every lane alike: no divergent
+        // branch in the loop). Only the middle row of the window needs them.
+        struct Halo {
             float ul, ur, vl, vr, pl, pr;
         };
-        auto load_row = [&](std::uint32_t i, Row& w) {
+        auto load_row = [&](std::uint32_t i, Row& w) __attribute__((always_inline)) {
             // cached loads: the halo rows a tile shares with its neighbours are then served by the L2
             const std::size_t off = static_cast<std::size_t>(i) * ld;
-            F::widen(*reinterpret_cast<const typename F::Raw*>(U + off + j0), w.u);
-            F::widen(*reinterpret_cast<const typename F::Raw*>(V + off + j0), w.v);
-            F::widen(*reinterpret_cast<const typename F::Raw*>(P + off + j0), w.p);
-            if (!left_lane) {
-                w.ul = F::one(U[off + jl]);
-                w.vl = F::one(V[off + jl]);
-                w.pl = F::one(P[off + jl]);
-            }
-            if (!right_lane) {
-                w.ur = F::one(U[off + jr]);
-                w.vr = F::one(V[off + jr]);
-                w.pr = F::one(P[off + jr]);
-            }
+            w.u = *reinterpret_cast<const typename F::Raw*>(U + off + j0);
+            w.v = *reinterpret_cast<const typename F::Raw*>(V + off + j0);
+            w.p = *reinterpret_cast<const typename F::Raw*>(P + off + j0);
+        };
+        auto load_halo = [&](std::uint32_t i, Halo& h) __attribute__((always_inline)) {
+            const std::size_t off = static_cast<std::size_t>(i) * ld;
+            h.ul = F::one(U[off + jl]);
+            h.vl = F::one(V[off + jl]);
+            h.pl = F::one(P[off + jl]);
+            h.ur = F::one(U[off + jr]);
+            h.vr = F::one(V[off + jr]);
+            h.pr = F::one(P[off + jr]);
         };
         auto wrap = [&](std::uint32_t i) { return i >= ny ? i - ny : i; };
-        // the window: rows i - 1 (m), i (c), i + 1 (n) and, loaded one iteration ahead of its use, i + 2 (n2)
-        Row m{}, c{}, n{}, n2{};  // halo slots a lane never loads stay 0
-        load_row(i0 == 0 ? ny - 1 : i0 - 1, m);
-        load_row(i0, c);
-        load_row(wrap(i0 + 1), n);
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        for (std::uint32_t i = i0; i < i1; ++i) {
-            if (i + 1 < i1) load_row(wrap(i + 2), n2);  // in flight while row i is computed
-            // x neighbours of this work-item's columns: from the adjacent lanes, else from memory (loaded with the row)
-            float ul = from_left_lane(c.u[kVec - 1]), ur = from_right_lane(c.u[0]);
-            float vl = from_left_lane(c.v[kVec - 1]), vr = from_right_lane(c.v[0]);
-            float pl = from_left_lane(c.p[kVec - 1]), pr = from_right_lane(c.p[0]);
-            if (!left_lane) {
-                ul = c.ul;
-                vl = c.vl;
-                pl = c.pl;
+        // Row i of the tile from the window m (i - 1), c (i), n (i + 1) and row i's halo: the residual stored, its
+        // squares summed.
+        auto emit = [&](std::uint32_t i, const Row& wm, const Row& wc, const Row& wn, const Halo& h) __attribute__((always_inline)) {
+            float mu[kVec], mv[kVec], mp[kVec], cu[kVec], cv[kVec], cp[kVec], nu_[kVec], nv[kVec], np[kVec];
+            F::widen(wm.u, mu);
+            F::widen(wm.v, mv);
+            F::widen(wm.p, mp);
+            F::widen(wc.u, cu);
+            F::widen(wc.v, cv);
+            F::widen(wc.p, cp);
+            F::widen(wn.u, nu_);
+            F::widen(wn.v, nv);
+            F::widen(wn.p, np);
+            // x neighbours of this work-item's columns: from the adjacent lanes, else from memory (the halo)
+            float ul = from_left_lane(cu[kVec - 1]), ur = from_right_lane(cu[0]);
+            float vl = from_left_lane(cv[kVec - 1]), vr = from_right_lane(cv[0]);
+            float pl = from_left_lane(cp[kVec - 1]), pr = from_right_lane(cp[0]);
+            if (left_mem) {
+                ul = h.ul;
+                vl = h.vl;
+                pl = h.pl;
             }
-            if (!right_lane) {
-                ur = c.ur;
-                vr = c.vr;
-                pr = c.pr;
+            if (right_mem) {
+                ur = h.ur;
+                vr = h.vr;
+                pr = h.pr;
             }
             float ru[kVec], rv[kVec], rc[kVec];
 #pragma unroll
             for (int e = 0; e < kVec; ++e) {
-                const float uw = e == 0 ? ul : c.u[e - 1], ue = e == kVec - 1 ? ur : c.u[e + 1];
-                const float vw = e == 0 ? vl : c.v[e - 1], ve = e == kVec - 1 ? vr : c.v[e + 1];
-                const float pw = e == 0 ? pl : c.p[e - 1], pe = e == kVec - 1 ? pr : c.p[e + 1];
-                cell(k, c.u[e], ue, uw, n.u[e], m.u[e], c.v[e], ve, vw, n.v[e], m.v[e], pe, pw, n.p[e], m.p[e], ru[e], rv[e], rc[e]);
+                const float uw = e == 0 ? ul : cu[e - 1], ue = e == kVec - 1 ? ur : cu[e + 1];
+                const float vw = e == 0 ? vl : cv[e - 1], ve = e == kVec - 1 ? vr : cv[e + 1];
+                const float pw = e == 0 ? pl : cp[e - 1], pe = e == kVec - 1 ? pr : cp[e + 1];
+                cell(k, cu[e], ue, uw, nu_[e], mu[e], cv[e], ve, vw, nv[e], mv[e], pe, pw, np[e], mp[e], ru[e], rv[e], rc[e]);
             }
             if (live) {
                 using F4 = float __attribute__((ext_vector_type(4)));
@@ -221,9 +233,44 @@ public:
                     s2 = fma(static_cast<double>(rc[e]), static_cast<double>(rc[e]), s2);
                 }
             }
-            m = c;
-            c = n;
-            n = n2;
+        };
+        // A ring of five rows: while row i is computed from rows i - 1, i, i + 1, row i + 2 is in flight and row i + 3
+        // starts loading, so each row has two iterations to arrive. The halo of row i + 1 is issued one iteration ahead,
+        // BEFORE row i + 3: loads complete in order, so waiting for the halo never waits for the newer row.
+        // Invariant at the top of row i: r0 = row i - 1, r1 = row i, r2 = row i + 1, r3 = row i + 2, hn = halo of row i.
+        Row  r0{}, r1{}, r2{}, r3{}, r4{};
+        Halo hc{}, hn{};
+        load_row(i0 == 0 ? ny - 1 : i0 - 1, r0);
+        load_row(i0, r1);
+        load_row(wrap(i0 + 1), r2);
+        load_halo(i0, hn);
+        if (i0 + 1 < i1) load_row(wrap(i0 + 2), r3);
+        // Steady state: five rows per pass, every load in range, no exit inside (the wait counts stay exact). The loop is
+        // unrolled by the ring's length, so the rows never move between registers.
+        auto step = [&](std::uint32_t i, const Row& m, const Row& c, const Row& n, Row& next) __attribute__((always_inline)) {
+            hc = hn;
+            load_halo(i + 1, hn);
+            load_row(wrap(i + 3), next);  // row i + 3 is the middle row two iterations from now
+            emit(i, m, c, n, hc);
+        };
+        std::uint32_t i = i0;
+        for (; i + 7 <= i1; i += 5) {
+            step(i, r0, r1, r2, r4);
+            step(i + 1, r1, r2, r3, r0);
+            step(i + 2, r2, r3, r4, r1);
+            step(i + 3, r3, r4, r0, r2);
+            step(i + 4, r4, r0, r1, r3);
+        }
+        // The last rows (at most six): the ring shifts by register moves.
+        for (; i < i1; ++i) {
+            hc = hn;
+            if (i + 1 < i1) load_halo(i + 1, hn);
+            if (i + 2 < i1) load_row(wrap(i + 3), r4);
+            emit(i, r0, r1, r2, hc);
+            r0 = r1;
+            r1 = r2;
+            r2 = r3;
+            r3 = r4;
         }
         // the tile's partial: a balanced tree over the work-group (deterministic)
         const auto g = it.get_group();

@@ -381,13 +381,13 @@ static void run_form(bool phased) {
 
 int main() {
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::index_kernel<batch::BatchPlan>), sizeof(GAPlan), emulate_plan);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedGemver>), sizeof(GAGem),
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::RaggedGemver>), sizeof(GAGem),
                           emulate_gemver);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::UnitMap>), sizeof(GAMap), emulate_map);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedPass1>), sizeof(GAP1), emulate_pass1);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedReduce>), sizeof(GARed),
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::UnitMap>), sizeof(GAMap), emulate_map);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::RaggedPass1>), sizeof(GAP1), emulate_pass1);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::RaggedReduce>), sizeof(GARed),
                           emulate_reduce);
-    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, batch::RaggedPass2>), sizeof(GAP2), emulate_pass2);
+    mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::RaggedPass2>), sizeof(GAP2), emulate_pass2);
 
     run_form(false);
     run_form(true);

@@ -67,7 +67,10 @@ def compare(g, store, fields):
 
 
 @pytest.mark.parametrize("style", [FUSED, UNFUSED])
-@pytest.mark.parametrize("nx,ny,ty", [(64, 48, 16), (1024, 40, 16), (2048, 64, 7), (1032, 33, 32)])
+# nx = 8: two live lanes, both halos the periodic wrap; nx = 1032: the last wave of a row holds two live lanes; ty = 7 and
+# ty = 64 with a partial last strip: the five-row steady state, its tail, and a tile shorter than the steady state
+@pytest.mark.parametrize("nx,ny,ty", [(64, 48, 16), (1024, 40, 16), (2048, 64, 7), (1032, 33, 32), (8, 24, 16), (1032, 130, 64),
+                                      (4096, 12, 4)])
 def test_ns_small_against_fp64(style, nx, ny, ty):
     print("ns", style, nx, ny, ty, flush=True)
     g = Ns(style, nx, ny, ty=ty, seed=3)
@@ -88,9 +91,9 @@ def test_ns_small_against_fp64(style, nx, ny, ty):
         g.close()
 
 
-def test_ns_fused_equals_unfused_bitwise():
-    nx, ny = 512, 96
-    a, b = Ns(FUSED, nx, ny, seed=5), Ns(UNFUSED, nx, ny, seed=5)
+@pytest.mark.parametrize("nx,ny,ty", [(512, 96, 16), (1032, 200, 64)])
+def test_ns_fused_equals_unfused_bitwise(nx, ny, ty):
+    a, b = Ns(FUSED, nx, ny, seed=5, ty=ty), Ns(UNFUSED, nx, ny, seed=5, ty=ty)
     try:
         a.run()
         b.run()
@@ -149,14 +152,14 @@ def test_ns_full_size_f32():
         g.close()
 
 
-def test_ns_bf16_fields():
+@pytest.mark.parametrize("nx,ny,ty", [(2048, 2048, 16), (1032, 70, 64)])
+def test_ns_bf16_fields(nx, ny, ty):
     """u, v, p stored bf16 (rounded to nearest even from the f32 fields), f32 arithmetic, FP64 norms; the oracle is FP64 on
     the stored bf16 values."""
-    n = 2048
-    g = Ns(FUSED, n, n, store=1, seed=2)
+    g = Ns(FUSED, nx, ny, store=1, seed=2, ty=ty)
     try:
         g.run()
-        fields = oracle_fields(2, n, n)
+        fields = oracle_fields(2, nx, ny)
         stored = [to_bf16(f) for f in fields]
         for name, s in zip("uvp", stored):
             assert np.array_equal(g.read(name), s), name
