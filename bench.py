@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 METRIC = "fused GEMVER/NS-residual HBM GB/s (% of 8 TB/s) vs plain; batched instances/s"
 PEAK_GBS = 8000.0
 UNFUSED, FUSED, NAIVE, TWOPASS, TWOPASS_1 = range(5)
+REPLAY_GRAPH, REPLAY_DIRECT = 0, 1  # CS_REPLAY_GRAPH, CS_REPLAY_DIRECT
 
 
 def log(*a):
@@ -68,6 +69,9 @@ class Gemver:
         self.check(self.w.cs_gemver_time_stage(self.h, leaf, iters, C.byref(ms)))
         return ms.value
 
+    def set_replay(self, mode):
+        self.check(self.w.cs_gemver_set_replay(self.h, mode))
+
     def close(self):
         self.check(self.w.cs_gemver_destroy(self.h))
 
@@ -106,6 +110,9 @@ class Ns:
         self.check(self.w.cs_ns_time_launches(self.h, iters, ms, 16, C.byref(cnt)))
         return [ms[k] for k in range(cnt.value)]
 
+    def set_replay(self, mode):
+        self.check(self.w.cs_ns_set_replay(self.h, mode))
+
     def close(self):
         self.check(self.w.cs_ns_destroy(self.h))
 
@@ -131,21 +138,35 @@ def gbs(nbytes, ms):
 
 
 def bench_ns(extra, steps):
+    """The NS residual at 4096^2 and 8192^2 (f32 fields) and 8192^2 (bf16 fields). The fused form's tile rows are picked by
+    timing 32 and 64; its steps are timed with graph replay and with direct replay (the headline of each line)."""
     for n, store, tag in ((4096, 0, "ns4096_f32"), (8192, 0, "ns8192_f32"), (8192, 1, "ns8192_bf16")):
         cells = n * n
         fbytes = cells * (3 * (2 if store else 4) + 3 * 4)
-        rec = {}
-        for style, name in ((0, "fused"), (1, "unfused")):
-            if store == 1 and style == 1:
-                continue
-            m = Ns(style, n, store)
-            ms = timed_steps(m, steps, 3)
+        tried = {}
+        for ty in (32, 64):
+            m = Ns(0, n, store, ty)
+            tried[ty] = timed_steps(m, 20, 3)
+            m.close()
+        ty = min(tried, key=tried.get)
+        rec = {"tile_rows": ty, "tile_rows_tried_ms": {str(k): round(v, 4) for k, v in tried.items()}}
+        m = Ns(0, n, store, ty)
+        gms = timed_steps(m, steps, 3)
+        m.set_replay(REPLAY_DIRECT)
+        ms = timed_steps(m, steps, 3)
+        ln = m.launches(3)
+        m.close()
+        rec["fused"] = {"ms": round(ms, 4), "GBps": round(gbs(fbytes, ms), 1), "pct_of_peak": round(100 * gbs(fbytes, ms) / PEAK_GBS, 2),
+                        "launch_ms": [round(t, 4) for t in ln], "replay": "direct"}
+        rec["fused_graph_replay"] = {"ms": round(gms, 4), "GBps": round(gbs(fbytes, gms), 1)}
+        if store == 0:
+            m = Ns(1, n, store)
+            ums = timed_steps(m, steps, 3)
             ln = m.launches(3)
             m.close()
-            rec[name] = {"ms": round(ms, 4), "GBps": round(gbs(fbytes, ms), 1), "pct_of_peak": round(100 * gbs(fbytes, ms) / PEAK_GBS, 2),
-                         "launch_ms": [round(t, 4) for t in ln]}
-        if "unfused" in rec:
-            rec["fused_speedup_vs_unfused"] = round(rec["unfused"]["ms"] / rec["fused"]["ms"], 3)
+            rec["unfused"] = {"ms": round(ums, 4), "GBps": round(gbs(fbytes, ums), 1), "pct_of_peak": round(100 * gbs(fbytes, ums) / PEAK_GBS, 2),
+                              "launch_ms": [round(t, 4) for t in ln]}
+            rec["fused_speedup_vs_unfused"] = round(ums / ms, 3)
         rec["algorithmic_bytes"] = fbytes
         extra[tag] = rec
         log(f"{tag}: {rec}")
@@ -391,6 +412,9 @@ def main():
     style = TWOPASS if a.style == "twopass" else TWOPASS_1
 
     g = Gemver(style, n, R, rt2=a.rt2)
+    # graph replay first (reported beside the headline), then the headline: the captured launches re-issued on one stream
+    ms_graph = timed_steps(g, a.steps, a.warmup)
+    g.set_replay(REPLAY_DIRECT)
     ms = timed_steps(g, a.steps, a.warmup)
     launch_ms = g.launches(5)
     # the dominant kernel: pass 1 (reads A and u1 v1 u2 v2 y) or pass 2 (reads A and u1 v1 u2 v2 x, writes w), each timed
@@ -399,7 +423,7 @@ def main():
     g.close()
     fused_bytes = gemver_bytes_fused(n)
     value = gbs(fused_bytes, ms)
-    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step, {value:.1f} GB/s; launches (us): {[round(t * 1e3, 1) for t in launch_ms]}"
+    log(f"fused two-pass GEMVER N={n}: {ms * 1e3:.1f} us/step (graph replay {ms_graph * 1e3:.1f}), {value:.1f} GB/s; launches (us): {[round(t * 1e3, 1) for t in launch_ms]}"
         f"; passes back to back (us): {[round(t * 1e3, 1) for t in stage_ms] if stage_ms else None}")
 
     traffic = profiled_traffic(n, R, a.rt2)
@@ -416,6 +440,7 @@ def main():
                 "traffic": ktraffic, "kernel": kname, "algorithmic_bytes": kbytes, "avg_launch_ms": round(kms, 4)}
 
     extra = {"fused_ms": round(ms, 4), "fused_pct_of_peak": round(100 * value / PEAK_GBS, 2), "launch_ms": [round(t, 4) for t in launch_ms],
+             "graph_replay": {"ms": round(ms_graph, 4), "GBps": round(gbs(fused_bytes, ms_graph), 1)},
              "pass_ms_back_to_back": [round(t, 4) for t in stage_ms] if stage_ms else None, "traffic_bytes": traffic}
     if not a.quick:
         p = Gemver(UNFUSED, n, 64)
@@ -450,7 +475,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (counter-hash uniform[-1,1), seed 0, generated on the device)",
         "config": {"workload": f"GEMVER N={n} f32 fused two-pass (A {n * n * 4 / 2**30:.2f} GiB, read twice), alpha=1.5 beta=1.2",
-                   "n": n, "R": R, "pass2_rows": a.rt2, "style": a.style},
+                   "n": n, "R": R, "pass2_rows": a.rt2, "style": a.style,
+                   "replay": "direct (the captured run's three launches re-issued on one stream; graph replay in extra)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,

@@ -70,6 +70,7 @@ HIP_API = [
 WORKLOADS_API = [
     ("cs_gemver_create", _I, [C.POINTER(_P), _I, _I, _S, _S, _F, _F, _U32]),
     ("cs_gemver_set_tile", _I, [_P, _I]),
+    ("cs_gemver_set_replay", _I, [_P, _I]),
     ("cs_gemver_run", _I, [_P]),
     ("cs_gemver_wait", _I, [_P]),
     ("cs_gemver_read", _I, [_P, C.c_char_p, _P, _S]),
@@ -85,6 +86,7 @@ WORKLOADS_API = [
     ("cs_ns_write", _I, [_P, C.c_char_p, _P, _S]),
     ("cs_ns_time", _I, [_P, _I, C.POINTER(_F)]),
     ("cs_ns_time_launches", _I, [_P, _I, C.POINTER(_F), _I, C.POINTER(_I)]),
+    ("cs_ns_set_replay", _I, [_P, _I]),
     ("cs_ns_destroy", _I, [_P]),
     ("cs_chain_create", _I, [C.POINTER(_P), _I, _S, _S, _U32]),
     ("cs_chain_run", _I, [_P]),

@@ -181,6 +181,14 @@ int cs_gemver_stats(cs_gemver g, uint64_t* stats4) {
     });
 }
 
+int cs_gemver_set_replay(cs_gemver g, int mode) {
+    return wl::guard([&] {
+        if (!g) throw std::invalid_argument("cs_gemver_set_replay: null handle");
+        if (mode != CS_REPLAY_GRAPH && mode != CS_REPLAY_DIRECT) throw std::invalid_argument("gemver: replay mode is CS_REPLAY_GRAPH or CS_REPLAY_DIRECT");
+        model(*g).set_replay_mode(mode == CS_REPLAY_DIRECT ? cs::ReplayMode::direct : cs::ReplayMode::graph);
+    });
+}
+
 int cs_gemver_destroy(cs_gemver g) {
     return wl::guard([&] {
         if (!g) return;

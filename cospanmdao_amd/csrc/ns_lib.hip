@@ -202,6 +202,14 @@ int cs_ns_time_launches(cs_ns g, int iters, float* ms, int cap, int* count) {
     });
 }
 
+int cs_ns_set_replay(cs_ns g, int mode) {
+    return wl::guard([&] {
+        if (!g) throw std::invalid_argument("cs_ns_set_replay: null handle");
+        if (mode != CS_REPLAY_GRAPH && mode != CS_REPLAY_DIRECT) throw std::invalid_argument("ns: replay mode is CS_REPLAY_GRAPH or CS_REPLAY_DIRECT");
+        model(*g).set_replay_mode(mode == CS_REPLAY_DIRECT ? cs::ReplayMode::direct : cs::ReplayMode::graph);
+    });
+}
+
 int cs_ns_destroy(cs_ns g) {
     return wl::guard([&] { delete g; });
 }

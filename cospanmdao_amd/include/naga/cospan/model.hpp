@@ -128,7 +128,13 @@ private:
 /// Counters of a model's executor.
 struct ModelStats {
     std::uint64_t runs = 0, launches = 0, captures = 0, replays = 0;
+    std::uint64_t relaunches = 0;  ///< replays served as direct launches (ReplayMode::direct)
 };
+
+/// How a steady-state run is replayed once it has been captured. graph: one hipGraphLaunch. direct: when the captured
+/// run stayed on stream 0, its recorded launches are re-issued in order on stream 0 with no events (a graph launch costs
+/// a few microseconds more than a short chain of plain launches); a run that spans streams still replays the graph.
+enum class ReplayMode { graph, direct };
 
 class Model {
 public:
@@ -189,7 +195,10 @@ public:
         }
         detail::Executor& ex = m.exec;
         if (replay_enabled_ && ex.has_graph() && sig == graph_sig_) {
-            ex.replay();
+            if (replay_mode_ == ReplayMode::direct && ex.graph_single_stream())
+                ex.relaunch(recs);
+            else
+                ex.replay();
             after_graph(m, active);
             end_turn(m, active);
             return;
@@ -359,11 +368,15 @@ public:
             last_sig_.clear();
         }
     }
+    /// How captured runs are replayed (graph by default).
+    void       set_replay_mode(ReplayMode mode) { replay_mode_ = mode; }
+    ReplayMode replay_mode() const { return replay_mode_; }
     ModelStats stats() const {
         ModelStats s = stats_;
         s.launches   = meta_->exec.launches();
         s.captures   = meta_->exec.captures();
         s.replays    = meta_->exec.replays();
+        s.relaunches = meta_->exec.relaunches();
         return s;
     }
     /// The stream the model's graph and first chain run on (for external timing with events).
@@ -625,6 +638,7 @@ private:
     std::unique_ptr<System>            root_;
     bool                               set_up_         = false;
     bool                               replay_enabled_ = true;
+    ReplayMode                         replay_mode_    = ReplayMode::graph;
     std::vector<std::byte>             last_sig_, graph_sig_;
     ModelStats                         stats_;
 };

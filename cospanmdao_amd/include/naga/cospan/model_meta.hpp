@@ -17,7 +17,8 @@
 //
 // Graph capture: a run whose launches are byte-identical to the previous run's is captured on stream 0 (side streams
 // join through fork events and are joined back before the capture ends) and every later identical run replays the
-// graph: one hipGraphLaunch instead of one launch per leaf.
+// graph: one hipGraphLaunch instead of one launch per leaf. A captured run that stayed on stream 0 can instead be re-issued
+// as its recorded launches in order on stream 0 with no events (Model's ReplayMode::direct).
 
 #include <cstddef>
 #include <cstdint>
@@ -211,7 +212,9 @@ public:
         capturing_     = false;
         hip_check(cs_hip_graph_end(streams_[0], &g), "graph capture");
         if (graph_) cs_hip_graph_destroy(graph_);
-        graph_ = g;
+        graph_         = g;
+        single_stream_ = true;
+        for (int k = 1; k < kStreams; ++k) single_stream_ = single_stream_ && !joined_[k];
         for (int k = 0; k < kStreams; ++k) last_[k].reset();
         ++captures_;
     }
@@ -219,6 +222,17 @@ public:
         join_all_into(0);
         hip_check(cs_hip_graph_launch(graph_, streams_[0]), "graph replay");
         ++replays_;
+    }
+    /// Whether the captured run used stream 0 alone (no side stream joined the capture).
+    bool graph_single_stream() const { return graph_ != nullptr && single_stream_; }
+    /// The captured run re-issued as its launches, in order, on stream 0 with no events: what the graph holds when it is
+    /// single-stream (every dependency is stream order). `recs` are the run's records, identical to the captured ones.
+    void relaunch(const std::vector<LaunchRecord>& recs) {
+        join_all_into(0);
+        for (const LaunchRecord& r : recs)
+            if (!r.empty()) r.submit(streams_[0]);
+        ++replays_;
+        ++relaunches_;
     }
     bool has_graph() const { return graph_ != nullptr; }
     void drop_graph() {
@@ -229,6 +243,7 @@ public:
     std::uint64_t launches() const { return launches_; }
     std::uint64_t captures() const { return captures_; }
     std::uint64_t replays() const { return replays_; }
+    std::uint64_t relaunches() const { return relaunches_; }
 
 private:
     /// During a capture a side stream joins it through an event recorded on the capturing stream 0.
@@ -247,8 +262,8 @@ private:
     cs_hip_stream         streams_[kStreams] = {};
     EventPtr              last_[kStreams];
     std::uint64_t         use_[kStreams] = {};
-    std::uint64_t         clock_ = 0, launches_ = 0, captures_ = 0, replays_ = 0;
-    bool                  capturing_ = false;
+    std::uint64_t         clock_ = 0, launches_ = 0, captures_ = 0, replays_ = 0, relaunches_ = 0;
+    bool                  capturing_ = false, single_stream_ = false;
     bool                  joined_[kStreams] = {};
     std::vector<EventPtr> forks_;
     cs_hip_graph          graph_  = nullptr;

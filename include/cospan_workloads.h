@@ -55,6 +55,13 @@ int cs_gemver_time_launches(cs_gemver g, int iters, float* ms, int cap, int* cou
 int cs_gemver_time_stage(cs_gemver g, int leaf, int iters, float* ms);
 /* the model's executor counters: runs, kernel launches, graph captures, graph replays */
 int cs_gemver_stats(cs_gemver g, uint64_t* stats4);
+/* how steady-state runs are replayed once captured (no counterpart in the reference, whose runs are always submitted
+ * eagerly): CS_REPLAY_GRAPH (the default) one hipGraphLaunch per run; CS_REPLAY_DIRECT the captured launches re-issued in
+ * order on the model's stream with no events, when the capture stayed on that one stream (a run spanning streams still
+ * replays its graph). Same results either way. */
+#define CS_REPLAY_GRAPH 0
+#define CS_REPLAY_DIRECT 1
+int cs_gemver_set_replay(cs_gemver g, int mode);
 int cs_gemver_destroy(cs_gemver g);
 
 /* ---- incompressible Navier-Stokes residual ---------------------------------------------------------------------------
@@ -81,6 +88,8 @@ int cs_ns_read(cs_ns m, const char* name, void* host, size_t bytes);
 int cs_ns_write(cs_ns m, const char* name, const void* host, size_t bytes);
 int cs_ns_time(cs_ns m, int iters, float* ms_per_run);
 int cs_ns_time_launches(cs_ns m, int iters, float* ms, int cap, int* count);
+/* CS_REPLAY_GRAPH or CS_REPLAY_DIRECT, as cs_gemver_set_replay */
+int cs_ns_set_replay(cs_ns m, int mode);
 int cs_ns_destroy(cs_ns m);
 
 /* ---- compressible Navier-Stokes residual ----------------------------------------------------------------------------

@@ -332,9 +332,11 @@ static void check_outputs(cs::Model& m, const std::vector<std::uint32_t>& sizes,
     std::printf("%s: %zu instances, worst w rel-L2 %.2e\n", what, sizes.size(), worst);
 }
 
-/// One form of the model through eager runs, a capture, replays and a parameter change.
-static void run_form(bool phased) {
-    const char*         form  = phased ? "phased" : "persistent";
+/// One form of the model through eager runs, a capture, replays and a parameter change (replays as graph launches, or with
+/// ReplayMode::direct as the captured launches re-issued on stream 0).
+static void run_form(bool phased, cs::ReplayMode mode = cs::ReplayMode::graph) {
+    const bool          direct = mode == cs::ReplayMode::direct;
+    const char*         form   = phased ? (direct ? "phased, direct replays" : "phased") : "persistent";
     const std::uint32_t count = 24, seed = 1;
     std::vector<std::uint32_t> sizes(count);
     or_ragged_sizes(seed, count, 16, 700, sizes.data());
@@ -350,6 +352,7 @@ static void run_form(bool phased) {
     batch::count_units(sizes.data(), count, 32, 4, knobs->units);
     cs::Model m = cs::create_model<batch::BatchedGemver>(layout, knobs, params, alpha, beta, seed, host_a_init(seed * 16 + 1), phased);
     m.setup();
+    m.set_replay_mode(mode);
     char what[160];
 
     m.run();
@@ -361,6 +364,9 @@ static void run_form(bool phased) {
     const cs::ModelStats st1 = m.stats();
     CHECK(st1.captures >= 1 && st1.replays >= 2, "%s: captures %llu replays %llu", form, (unsigned long long)st1.captures,
           (unsigned long long)st1.replays);
+    // the batched model runs on one stream: direct replays are plain launches, graph replays one graph launch each
+    CHECK(direct ? st1.relaunches == st1.replays - 1 : st1.relaunches == 0, "%s: relaunches %llu of %llu replays", form,
+          (unsigned long long)st1.relaunches, (unsigned long long)st1.replays);
 
     params        = {64, 8, 17};  // new parameters: new counts, a new grid
     knobs->groups = 17;
@@ -391,6 +397,7 @@ int main() {
 
     run_form(false);
     run_form(true);
+    run_form(true, cs::ReplayMode::direct);
 
     const std::string f = mock::failure();
     CHECK(f.empty(), "mock: %s", f.c_str());

@@ -3,7 +3,9 @@
 // Builds workloads/chain_systems.hpp's Chain in each of its six styles through the real Model / System / Executor code,
 // linked against the CPU implementation of the C boundary (mock_hip.cpp) with a no-op stand-in for every kernel: it checks
 // the wiring (every input has a source or an initial value, every shape resolves) and the number of launches per run
-// against the styles' plans (examples/gemver-chain/systems.hpp:33-53), eager and replayed. Prints "OK" when every check holds.
+// against the styles' plans (examples/gemver-chain/systems.hpp:33-53), eager and replayed, and which replays ReplayMode::direct
+// serves as plain launches (single-stream captures) or as graph launches (the parallel style). Prints "OK" when every check
+// holds.
 #include <cstdio>
 #include <string>
 
@@ -58,6 +60,29 @@ int main() {
                 CHECK(shape.size() == 2 && shape[0] == 512 && shape[1] == 512, "%s: '%s' has a wrong shape", c.name, v);
             }
             std::printf("%s: %d launches per chain, captured and replayed\n", c.name, c.launches);
+
+            // ReplayMode::direct: a capture that stayed on stream 0 is re-issued as its launches (no graph launch); one that
+            // spans streams (the parallel style's independent components) still replays its graph
+            cs::Model d = cs::create_model<chain::Chain>(c.style, 512, 64, 0u);
+            d.setup();
+            d.set_replay_mode(cs::ReplayMode::direct);
+            for (int i = 0; i < 3; ++i) d.run();  // eager, captured, replayed
+            d.wait();
+            const mock::Stats    s0 = mock::stats();
+            const std::uint64_t  r0 = d.stats().relaunches;
+            for (int i = 0; i < 2; ++i) d.run();  // replays
+            d.wait();
+            const mock::Stats    s1     = mock::stats();
+            const std::uint64_t  r1     = d.stats().relaunches;
+            const std::uint64_t  plain  = s1.launches - s0.launches, graphs = s1.graph_launches - s0.graph_launches;
+            const bool           multi  = graphs != 0;
+            // every replay is one or the other, whole: the chain's launches on stream 0, or one graph launch
+            CHECK((r1 - r0 == 2 && graphs == 0 && static_cast<int>(plain) == 2 * c.launches) || (r1 == r0 && graphs == 2 && plain == 0),
+                  "%s direct: relaunches %llu, graph launches %llu, launches %llu", c.name, (unsigned long long)(r1 - r0),
+                  (unsigned long long)graphs, (unsigned long long)plain);
+            // the parallel style runs its independent components on side streams: its capture spans streams
+            CHECK(c.style != chain::Style::parallel || multi, "%s direct: the parallel style's capture should span streams", c.name);
+            std::printf("%s direct: %s\n", c.name, multi ? "multi-stream capture, graph replays" : "relaunched on stream 0");
         } catch (const std::exception& e) {
             CHECK(false, "%s: %s", c.name, e.what());
         }

@@ -145,6 +145,32 @@ def test_twopass_reduced_precision(store, n):
         g.close()
 
 
+def test_direct_replay_matches_graph_replay():
+    """CS_REPLAY_DIRECT re-issues the captured launches on the model's stream instead of launching the graph: the same x and
+    w, bit for bit, over eager, captured and replayed runs, with and without a wait between runs."""
+    n = 1024
+    a, b = Gemver(TWOPASS, n, 128, seed=3), Gemver(TWOPASS, n, 128, seed=3)
+    try:
+        check(workloads.cs_gemver_set_replay(b.h, 1))
+        for i in range(6):
+            if i == 4:  # three runs back to back, one wait
+                for _ in range(3):
+                    check(workloads.cs_gemver_run(a.h))
+                    check(workloads.cs_gemver_run(b.h))
+            a.run()
+            b.run()
+            for name in ("x", "w"):
+                assert np.array_equal(a.read(name, np.float32, n), b.read(name, np.float32, n)), (i, name)
+        for g in (a, b):
+            st = (C.c_uint64 * 4)()
+            check(workloads.cs_gemver_stats(g.h, st))
+            assert st[2] == 1 and st[3] >= 6, list(st)  # one capture, the later runs replayed (by either mechanism)
+        assert workloads.cs_gemver_set_replay(b.h, 7) != 0
+    finally:
+        a.close()
+        b.close()
+
+
 def test_bad_arguments_are_errors():
     h = C.c_void_p()
     assert workloads.cs_gemver_create(C.byref(h), UNFUSED, 0, 1000, 64, 1.0, 1.0, 0) == 1  # n % 256 != 0

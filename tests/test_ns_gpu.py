@@ -121,6 +121,32 @@ def test_ns_uniform_field_gives_exactly_zero(style):
         g.close()
 
 
+def test_ns_direct_replay_matches_graph_replay():
+    """CS_REPLAY_DIRECT: the same residuals and norms as graph replay, bit for bit, across a field update (update_input keeps
+    the capture) and runs back to back."""
+    nx, ny = 1024, 136
+    a, b = Ns(FUSED, nx, ny, seed=4, ty=64), Ns(FUSED, nx, ny, seed=4, ty=64)
+    try:
+        check(workloads.cs_ns_set_replay(b.h, 1))
+        u = np.random.default_rng(7).uniform(-1, 1, nx * ny).astype(np.float32)
+        for i in range(6):
+            if i == 3:
+                a.write("u", u)
+                b.write("u", u)
+            if i == 4:
+                for _ in range(3):
+                    check(workloads.cs_ns_run(a.h))
+                    check(workloads.cs_ns_run(b.h))
+            a.run()
+            b.run()
+            for k in ("Ru", "Rv", "Rc", "sq", "norm"):
+                assert np.array_equal(a.read(k), b.read(k)), (i, k)
+        assert workloads.cs_ns_set_replay(b.h, -1) != 0
+    finally:
+        a.close()
+        b.close()
+
+
 def test_ns_graph_replay_keeps_results():
     nx, ny = 1024, 128
     g = Ns(FUSED, nx, ny)
