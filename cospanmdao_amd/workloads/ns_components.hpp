@@ -322,7 +322,7 @@ private:
 };
 
 // =============================================================================
-// NsNorm: the fixed-order reduction of per-tile partials (one work-group)
+// NsNorm: the fixed-order reduction of per-tile partials (one wave)
 // =============================================================================
 class NsNorm : public cs::Component<NsNorm> {
 public:
@@ -334,14 +334,30 @@ public:
         cs::Output<DType::f64, 1> sq;    ///< the three sums of squares
         cs::Output<DType::f64, 1> norm;  ///< ||Ru||, ||Rv||, ||Rc||, and the norm of all three
     };
+    /// Up to kOneWave partials (the fused tile's), one wave does it all and the other waves return at once: lane l sums
+    /// partials l, l + 64, ... in order, then a butterfly over the lanes; no LDS, no barrier. More partials (the unfused
+    /// variant's one per 256 cells): every work-item sums a stride of 256, each wave reduces by butterfly, and the four
+    /// wave results meet through LDS after one barrier, summed pairwise in wave order. Both orders are fixed.
+    static constexpr std::uint32_t kOneWave = 1024;
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         const std::uint32_t tiles = u32(in.part.shape()[0]);
-        const std::uint32_t tid   = u32(it.get_local_id(0)), nt = u32(it.get_local_range(0));
-        double              s[3]  = {0.0, 0.0, 0.0};
+        const std::uint32_t tid   = u32(it.get_local_id(0));
+        const std::uint32_t nt    = tiles <= kOneWave ? static_cast<std::uint32_t>(cs::wave::kSize) : static_cast<std::uint32_t>(kWg);
+        if (tid >= nt) return;  // uniform: the branch depends on the partial count only
+        double s[3] = {0.0, 0.0, 0.0};
         for (std::uint32_t t = tid; t < tiles; t += nt)
             for (int c = 0; c < 3; ++c) s[c] += in.part[cs::id<2>{t, static_cast<std::size_t>(c)}];
-        const auto g = it.get_group();
-        for (int c = 0; c < 3; ++c) s[c] = cs::reduce_over_group(g, s[c], cs::plus<double>());
+        for (int c = 0; c < 3; ++c) s[c] = cs::wave::reduce(s[c], cs::plus<double>());
+        if (nt == static_cast<std::uint32_t>(kWg)) {
+            constexpr int     kWaves = kWg / cs::wave::kSize;
+            __shared__ double wave_sums[kWaves][3];
+            const int         wv = static_cast<int>(tid) / cs::wave::kSize;
+            if (cs::wave::lane_id() == 0)
+                for (int c = 0; c < 3; ++c) wave_sums[wv][c] = s[c];
+            __syncthreads();
+            static_assert(kWaves == 4, "cospan ns: the norm's work-group is four waves");
+            for (int c = 0; c < 3; ++c) s[c] = (wave_sums[0][c] + wave_sums[1][c]) + (wave_sums[2][c] + wave_sums[3][c]);
+        }
         if (tid == 0) {
             for (int c = 0; c < 3; ++c) {
                 out.sq[cs::id<1>{static_cast<std::size_t>(c)}]   = s[c];
