@@ -218,8 +218,8 @@ public:
         for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
             float        acc = 0.0f;
             const float* pj  = P + j;
-            // the sum stays in ascending c; unrolling keeps 16 independent loads in flight per work-item
-#pragma unroll 16
+            // the sum stays in ascending c; unrolling keeps 32 independent loads in flight per work-item
+#pragma unroll 32
             for (std::uint32_t c = 0; c < strips; ++c) acc += pj[static_cast<std::size_t>(c) * ld];
             if constexpr (Z)
                 out.x[cs::id<1>{j}] = fmaf(scale, acc, in.z[cs::id<1>{j}]);
@@ -230,6 +230,10 @@ public:
 
     Inputs  inputs;
     Outputs outputs;
+
+    /// `wg`: work-items per work-group without the dots (kWg inside a fused group, whose stages share one shape; one
+    /// wave as a launch of its own, which spreads the columns over every CU: the reduction is latency-bound)
+    explicit ColReduce(int wg = kWg) : wg_{wg} {}
 
     void declare() override {
         this->declare_input(inputs.P, "P", cs::unspecified_shape<2>());
@@ -248,9 +252,13 @@ public:
         if constexpr (Dots) this->set_shape(outputs.dots, 2, round_up(n, kWg) / kWg);
     }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        const std::size_t range = inputs.P.shape()[1];
-        return this->create_launch_config(cs::range_desc(cs::range<1>{round_up(range, kWg)}), cs::wg_desc(cs::range<1>{kWg}));
+        // one column per work-item; the dots' partials are per work-group of kWg
+        const std::size_t range = inputs.P.shape()[1], wg = Dots ? kWg : static_cast<std::size_t>(wg_);
+        return this->create_launch_config(cs::range_desc(cs::range<1>{round_up(range, wg)}), cs::wg_desc(cs::range<1>{wg}));
     }
+
+private:
+    int wg_;
 };
 
 class Gemv : public cs::Component<Gemv> {

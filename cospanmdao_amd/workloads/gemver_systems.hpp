@@ -109,7 +109,7 @@ public:
                 pre = "fw.";
             } else {
                 add_subsystem(cs::subsystem_desc<AhatTPartial<S>>("p1", R_));
-                add_subsystem(cs::subsystem_desc<Cred>("cred"));
+                add_subsystem(cs::subsystem_desc<Cred>("cred", cs::wave::kSize));  // a launch of its own: one-wave work-groups
                 add_subsystem(cs::subsystem_desc<AhatGemv<S, RT>>("p2"));
                 connect("p1.P", "cred.P");
                 connect("cred.x", "p2.x");
