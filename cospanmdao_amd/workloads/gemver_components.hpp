@@ -272,6 +272,9 @@ public:
     struct Outputs {
         cs::Output<DType::f32, 1> y;
     };
+    /// Rows per work-group: each row keeps the reference's order (work-item l: one fma chain over j = l, l + 256, ...; then
+    /// the group's tree), and x[j] is loaded once for the work-group's rows instead of once per row.
+    static constexpr int kRows = 4;
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         const std::uint32_t n     = u32(in.B.shape()[0]);
         const float         scale = in.scale[std::size_t{0}];
@@ -279,14 +282,27 @@ public:
         const std::uint32_t g = u32(it.get_group(0)), ng = u32(it.get_group_range(0));
         const std::size_t ldb = in.B.stride()[0];
         const float*      x   = cs::raw::data(in.x);
-        for (std::uint32_t i = g; i < n; i += ng) {
-            const float* row = cs::raw::data(in.B) + static_cast<std::size_t>(i) * ldb;
-            float        acc = 0.0f;
-            // the loads of 8 steps in flight; the sum stays one fma chain over j = l, l + wg, ... (the order is the spec)
-#pragma unroll 8
-            for (std::uint32_t j = l; j < n; j += wg) acc = fmaf(row[j], x[j], acc);
-            const float sum = cs::reduce_over_group(it.get_group(), acc, cs::plus<float>());
-            if (l == 0) out.y[cs::id<1>{i}] = scale * sum;
+        for (std::uint32_t i0 = g * kRows; i0 < n; i0 += ng * kRows) {
+            const float* row[kRows];
+            float        acc[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const std::uint32_t i = i0 + static_cast<std::uint32_t>(r) < n ? i0 + static_cast<std::uint32_t>(r) : i0;
+                row[r]                = cs::raw::data(in.B) + static_cast<std::size_t>(i) * ldb;
+                acc[r]                = 0.0f;
+            }
+            // the loads of 4 steps of the rows in flight; each row's sum stays one fma chain over j = l, l + wg, ...
+#pragma unroll 4
+            for (std::uint32_t j = l; j < n; j += wg) {
+                const float xj = x[j];
+#pragma unroll
+                for (int r = 0; r < kRows; ++r) acc[r] = fmaf(row[r][j], xj, acc[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const float sum = cs::reduce_over_group(it.get_group(), acc[r], cs::plus<float>());
+                if (l == 0 && i0 + static_cast<std::uint32_t>(r) < n) out.y[cs::id<1>{i0 + static_cast<std::uint32_t>(r)}] = scale * sum;
+            }
         }
     }
 
@@ -301,7 +317,7 @@ public:
     }
     void prepare_forward() override { set_shape(outputs.y, shape_of(inputs.B)[0]); }
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
-        const std::size_t range = inputs.B.shape()[0] * kWg;  // a group per row
+        const std::size_t range = (inputs.B.shape()[0] + kRows - 1) / kRows * kWg;  // a group per kRows rows
         return create_launch_config(cs::range_desc(cs::range<1>{range}), cs::wg_desc(cs::range<1>{kWg}));
     }
 };
