@@ -218,8 +218,10 @@ public:
         for (std::uint32_t j = u32(it.get_global_id(0)); j < n; j += stride) {
             float        acc = 0.0f;
             const float* pj  = P + j;
-            // the sum stays in ascending c; unrolling keeps 32 independent loads in flight per work-item
-#pragma unroll 32
+            // the sum stays in ascending c; unrolling keeps 64 independent loads in flight per work-item (the hardware's
+            // limit of outstanding loads per wave is 63): a column of 1024 strips (16-row strips at n = 16384) is 16
+            // rounds of loads
+#pragma unroll 64
             for (std::uint32_t c = 0; c < strips; ++c) acc += pj[static_cast<std::size_t>(c) * ld];
             if constexpr (Z)
                 out.x[cs::id<1>{j}] = fmaf(scale, acc, in.z[cs::id<1>{j}]);
