@@ -30,6 +30,7 @@ def _load(name):
 # (name, restype, argtypes) of every entry point of include/cospan_hip.h
 HIP_API = [
     ("cs_hip_last_error", _I, [C.c_char_p, _S]),
+    ("cs_hip_build_id", C.c_char_p, []),
     ("cs_hip_init", _I, [_I]),
     ("cs_hip_device_count", _I, [C.POINTER(_I)]),
     ("cs_hip_device_info", _I, [_I, C.c_char_p, _S, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)]),
@@ -58,6 +59,10 @@ HIP_API = [
     ("cs_hip_graph_launch", _I, [_P, _P]),
     ("cs_hip_graph_destroy", _I, [_P]),
     ("cs_hip_stream_capturing", _I, [_P, C.POINTER(_I)]),
+    ("cs_hip_set_spin_limit", _I, [_U32]),
+    ("cs_hip_spin_limit", _U32, []),
+    ("cs_hip_fault_word", _I, [C.POINTER(_P)]),
+    ("cs_hip_take_fault", _I, [C.POINTER(_U32)]),
     ("cs_hip_scan_workspace_bytes", _S, [_S]),
     ("cs_hip_scan_workspace_init", _I, [_P, _S, _P]),
     ("cs_hip_exclusive_scan_u32", _I, [_P, _P, _S, _P, _P]),
@@ -68,6 +73,7 @@ HIP_API = [
 ]
 
 WORKLOADS_API = [
+    ("cs_workloads_build_id", C.c_char_p, []),
     ("cs_gemver_create", _I, [C.POINTER(_P), _I, _I, _S, _S, _F, _F, _U32]),
     ("cs_gemver_set_tile", _I, [_P, _I]),
     ("cs_gemver_set_replay", _I, [_P, _I]),

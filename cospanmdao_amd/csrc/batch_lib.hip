@@ -104,6 +104,10 @@ struct cs_batch_s {
 
 namespace {
 
+/// Whether cs_batch_set_mode may select the persistent work-queue form (RaggedGemver). Off until its restructured work
+/// loop has passed its GPU tests: an earlier form of the loop hung the GPU.
+constexpr bool kPersistentFormEnabled = false;
+
 cs::Model& model(cs_batch_s& g) {
     if (!g.model) {
         g.knobs->groups = static_cast<int>(g.params[2]);
@@ -204,6 +208,11 @@ int cs_batch_set_mode(cs_batch g, int persistent) {
     return wl::guard([&] {
         if (!g) throw std::invalid_argument("cs_batch_set_mode: null handle");
         if (persistent != 0 && persistent != 1) throw std::invalid_argument("cs_batch_set_mode: mode is 0 (phased) or 1 (persistent)");
+        if (persistent == 1 && !kPersistentFormEnabled)
+            throw std::invalid_argument(
+                "cs_batch_set_mode: the persistent form is switched off: its work loop hung on the GPU (a leader-only "
+                "completion and the next leader-only fetch merged across the loop's back edge); the restructured loop has "
+                "not yet run on a GPU");
         if ((persistent != 0) != g->persistent) {
             if (g->model) g->model->wait();
             g->model.reset();  // rebuilt in the other form at the next use

@@ -13,5 +13,7 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int check(hipError_t e, const char* what);
 /// The HIP stream behind a handle (null: the null stream).
 hipStream_t raw(cs_hip_stream s);
+/// The device address of the current device's fault word (looked up once per device).
+int fault_word(uint32_t** out);
 
 }  // namespace cs_hip

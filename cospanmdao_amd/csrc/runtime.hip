@@ -5,11 +5,13 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "cospan_hip.h"
 #include "internal.hpp"
@@ -49,8 +51,34 @@ struct cs_hip_graph_s {
     hipGraphExec_t exec;
 };
 
+namespace {
+
+/// The fault word of each device (every device holds its own copy of this variable at its own address). Kernels OR
+/// CS_HIP_FAULT_* bits into it when a bounded wait gives up; the host reads and clears it after the work completed.
+__device__ uint32_t g_fault_word;
+
+std::atomic<uint32_t>  g_spin_limit{CS_HIP_DEFAULT_SPIN_LIMIT};
+std::mutex             g_fault_mu;
+std::vector<uint32_t*> g_fault_ptr;  // per device: the address of g_fault_word there
+
+}  // namespace
+
 namespace cs_hip {
 hipStream_t raw(cs_hip_stream s) { return s ? s->s : nullptr; }
+
+int fault_word(uint32_t** out) {
+    int dev = 0;
+    if (int r = check(hipGetDevice(&dev), "hipGetDevice")) return r;
+    std::lock_guard<std::mutex> lock{g_fault_mu};
+    if (g_fault_ptr.size() <= static_cast<size_t>(dev)) g_fault_ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+    if (!g_fault_ptr[dev]) {
+        void* p = nullptr;
+        if (int r = check(hipGetSymbolAddress(&p, HIP_SYMBOL(g_fault_word)), "hipGetSymbolAddress(fault word)")) return r;
+        g_fault_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_fault_ptr[dev];
+    return CS_HIP_OK;
+}
 }  // namespace cs_hip
 
 extern "C" {
@@ -246,6 +274,32 @@ int cs_hip_graph_destroy(cs_hip_graph g) {
     if (int r2 = check(hipGraphDestroy(g->g), "hipGraphDestroy")) r = r ? r : r2;
     delete g;
     return r;
+}
+
+int cs_hip_set_spin_limit(uint32_t limit) {
+    g_spin_limit.store(limit, std::memory_order_relaxed);
+    return CS_HIP_OK;
+}
+
+uint32_t cs_hip_spin_limit(void) { return g_spin_limit.load(std::memory_order_relaxed); }
+
+int cs_hip_fault_word(uint32_t** device_word) {
+    if (!device_word) return fail(CS_HIP_ERR_INVALID, "cs_hip_fault_word: null");
+    return cs_hip::fault_word(device_word);
+}
+
+int cs_hip_take_fault(uint32_t* bits) {
+    if (!bits) return fail(CS_HIP_ERR_INVALID, "cs_hip_take_fault: null");
+    uint32_t* p = nullptr;
+    if (int r = cs_hip::fault_word(&p)) return r;
+    uint32_t v = 0;
+    if (int r = check(hipMemcpy(&v, p, sizeof v, hipMemcpyDeviceToHost), "hipMemcpy(fault word)")) return r;
+    if (v != 0) {
+        const uint32_t zero = 0;
+        if (int r = check(hipMemcpy(p, &zero, sizeof zero, hipMemcpyHostToDevice), "hipMemcpy(fault word)")) return r;
+    }
+    *bits = v;
+    return CS_HIP_OK;
 }
 
 int cs_hip_stream_capturing(cs_hip_stream s, int* capturing) {

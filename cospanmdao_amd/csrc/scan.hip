@@ -17,7 +17,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "cospan_hip.h"
@@ -33,7 +35,6 @@ constexpr int      kItems   = 8;
 constexpr int      kTile    = kThreads * kItems;
 constexpr uint64_t kStateAgg  = 1ull << 62;
 constexpr uint64_t kStateIncl = 2ull << 62;
-constexpr uint32_t kSpinLimit = 1u << 24;
 
 /// The workspace: a header, then one status word per tile (state in bits 63..62, value in bits 31..0).
 struct ScanHeader {
@@ -41,6 +42,12 @@ struct ScanHeader {
     uint32_t done;
     uint32_t flag;
     uint32_t pad;
+};
+
+/// Where a wait that gives up reports, and how often a wait polls (cs_hip_spin_limit at launch).
+struct WaitBound {
+    uint32_t* fault;
+    uint32_t  limit;
 };
 
 __device__ inline uint32_t atomic_load_u32(uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -99,7 +106,7 @@ __device__ inline void finish(ScanHeader* ws, uint64_t* status, uint32_t tiles, 
 template <bool Column, class Src>
 __global__ __launch_bounds__(kThreads) void scan_kernel(const Src* __restrict__ in, uint64_t kstride, uint32_t* __restrict__ extents,
                                                         uint32_t* __restrict__ out, uint64_t n, uint32_t tiles, ScanHeader* ws,
-                                                        uint32_t* result) {
+                                                        uint32_t* result, WaitBound wb) {
     __shared__ uint32_t s_items[kTile];
     __shared__ uint32_t s_tile, s_prefix, s_flag;
     uint64_t* const     status = reinterpret_cast<uint64_t*>(ws + 1);
@@ -111,7 +118,11 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const Src* __restrict__ 
     }
     __syncthreads();
     const uint32_t tile = s_tile;
-    if (tile >= tiles) return;  // only with a workspace that was not zeroed: never write outside it
+    if (tile >= tiles) {
+        // only with a workspace that was not zeroed (two scans sharing it): never write outside it, and say so
+        if (tid == 0) cs::bounded::report(wb.fault, CS_HIP_FAULT_STATE);
+        return;
+    }
     const uint64_t base = static_cast<uint64_t>(tile) * kTile;
 
     // striped, coalesced loads into LDS (the narrowing prologue of a column build on the way)
@@ -153,14 +164,17 @@ __global__ __launch_bounds__(kThreads) void scan_kernel(const Src* __restrict__ 
             int64_t end = tile;
             while (true) {
                 const int64_t p  = end - 1 - tid;
-                uint64_t      st = kStateIncl;
+                uint64_t      st = kStateIncl;  // before tile 0: an inclusive prefix of 0
                 if (p >= 0) {
                     // every predecessor has started (tickets in start order) and publishes without waiting; the bound
-                    // only guarantees that a corrupted workspace cannot hang the device
-                    uint32_t spins = 0;
-                    do {
-                        st = atomic_load_u64(&status[p]);
-                    } while ((st >> 62) == 0 && ++spins < kSpinLimit && (__builtin_amdgcn_s_sleep(1), true));
+                    // only guarantees that a corrupted workspace can neither hang the device nor pass unnoticed (a
+                    // lane that gives up counts the tile as 0 and the look-back walks on)
+                    st = 0;
+                    if (!cs::bounded::poll([&] { return ((st = atomic_load_u64(&status[p])) >> 62) != 0; }, wb.limit)) {
+                        st = 0;
+                        cs::bounded::report(wb.fault, CS_HIP_FAULT_SCAN_LOOKBACK);
+                        atomicOr(&s_flag, CS_HIP_FLAG_WAIT_TIMEOUT);
+                    }
                 }
                 const uint64_t incl = cs::wave::ballot((st >> 62) == 2);
                 uint32_t       v    = static_cast<uint32_t>(st);
@@ -238,20 +252,25 @@ __global__ __launch_bounds__(kThreads) void offsets_kernel(const Src* __restrict
     finish(ws, nullptr, groups, s_flag, result, total, false);
 }
 
-std::mutex g_ws_mu;
+/// The library-owned workspaces: one per (device, stream), so scans on different streams never share one; scans on one
+/// stream run in order. An outgrown workspace is retired, not freed: a captured graph may still name it.
 struct OwnedWorkspace {
     void*  ptr   = nullptr;
     size_t bytes = 0;
 };
-std::vector<OwnedWorkspace> g_ws;  // one per device
+std::mutex                                             g_ws_mu;
+std::map<std::pair<int, hipStream_t>, OwnedWorkspace> g_ws;
+std::vector<void*>                                     g_ws_retired;
 
-/// The library-owned workspace of the current device, grown (outside any capture) to `need` bytes.
-int owned_workspace(size_t need, cs_hip_stream s, void** out) {
+/// Runs `launch(workspace)` with the library-owned workspace of the current device and stream `s`, grown (outside any
+/// capture) to `need` bytes. The lock is held through the launch, so no other thread can grow the workspace between
+/// the lookup and the enqueue.
+template <class Launch>
+int with_owned_workspace(size_t need, cs_hip_stream s, Launch&& launch) {
     int dev = 0;
     if (int r = cs_hip::check(hipGetDevice(&dev), "hipGetDevice")) return r;
     std::lock_guard<std::mutex> lock{g_ws_mu};
-    if (g_ws.size() <= static_cast<size_t>(dev)) g_ws.resize(dev + 1);
-    OwnedWorkspace& w = g_ws[dev];
+    OwnedWorkspace&             w = g_ws[{dev, cs_hip::raw(s)}];
     if (w.bytes < need) {
         hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
         if (int r = cs_hip::check(hipStreamIsCapturing(cs_hip::raw(s), &st), "hipStreamIsCapturing")) return r;
@@ -260,20 +279,22 @@ int owned_workspace(size_t need, cs_hip_stream s, void** out) {
                                 "scan: the library workspace must grow to %zu bytes, which cannot happen during graph capture; "
                                 "run the scan once before capturing or pass a workspace",
                                 need);
-        if (w.ptr) {
-            if (int r = cs_hip::check(hipDeviceSynchronize(), "hipDeviceSynchronize")) return r;
-            if (int r = cs_hip::check(hipFree(w.ptr), "hipFree")) return r;
-            w.ptr   = nullptr;
-            w.bytes = 0;
-        }
-        size_t bytes = need < 4096 ? 4096 : need * 2;
-        if (int r = cs_hip::check(hipMalloc(&w.ptr, bytes), "hipMalloc")) return r;
-        if (int r = cs_hip::check(hipMemset(w.ptr, 0, bytes), "hipMemset")) return r;
-        if (int r = cs_hip::check(hipDeviceSynchronize(), "hipDeviceSynchronize")) return r;
+        const size_t bytes = (need < 4096 ? 4096 : need * 2 + 3) / 4 * 4;
+        void*        p     = nullptr;
+        if (int r = cs_hip::check(hipMalloc(&p, bytes), "hipMalloc")) return r;
+        // zeroed on the scan's own stream, ahead of the scan (a kernel, not a memset)
+        if (int r = cs_hip_fill_u32_async(p, 0u, bytes / 4, s)) return r;
+        if (w.ptr) g_ws_retired.push_back(w.ptr);
+        w.ptr   = p;
         w.bytes = bytes;
     }
-    *out = w.ptr;
-    return CS_HIP_OK;
+    return launch(w.ptr);
+}
+
+WaitBound wait_bound() {
+    WaitBound b{nullptr, cs_hip_spin_limit()};
+    if (cs_hip::fault_word(&b.fault) != CS_HIP_OK) b.fault = nullptr;
+    return b;
 }
 
 uint32_t tiles_for(size_t n) { return static_cast<uint32_t>(n == 0 ? 1 : (n + kTile - 1) / kTile); }
@@ -281,23 +302,26 @@ uint32_t tiles_for(size_t n) { return static_cast<uint32_t>(n == 0 ? 1 : (n + kT
 template <bool Column, class Src>
 int launch_scan(const Src* in, size_t kstride, uint32_t* extents, uint32_t* out, size_t n, uint32_t* result, void* workspace,
                 cs_hip_stream s) {
-    const uint32_t tiles = tiles_for(n);
-    if (!workspace)
-        if (int r = owned_workspace(cs_hip_scan_workspace_bytes(n), s, &workspace)) return r;
-    hipLaunchKernelGGL((scan_kernel<Column, Src>), dim3(tiles), dim3(kThreads), 0, cs_hip::raw(s), in, static_cast<uint64_t>(kstride),
-                       extents, out, static_cast<uint64_t>(n), tiles, static_cast<ScanHeader*>(workspace), result);
-    return cs_hip::check(hipGetLastError(), "scan launch");
+    const uint32_t  tiles = tiles_for(n);
+    const WaitBound wb    = wait_bound();
+    auto            go    = [&](void* ws) {
+        hipLaunchKernelGGL((scan_kernel<Column, Src>), dim3(tiles), dim3(kThreads), 0, cs_hip::raw(s), in, static_cast<uint64_t>(kstride),
+                           extents, out, static_cast<uint64_t>(n), tiles, static_cast<ScanHeader*>(ws), result, wb);
+        return cs_hip::check(hipGetLastError(), "scan launch");
+    };
+    return workspace ? go(workspace) : with_owned_workspace(cs_hip_scan_workspace_bytes(n), s, go);
 }
 
 template <class Src>
 int launch_offsets(const Src* in, size_t count_src, uint32_t* extents, uint32_t* offs, uint32_t* result, void* workspace,
                    cs_hip_stream s) {
     const uint32_t groups = static_cast<uint32_t>((count_src + kThreads - 1) / kThreads);
-    if (!workspace)
-        if (int r = owned_workspace(cs_hip_scan_workspace_bytes(count_src), s, &workspace)) return r;
-    hipLaunchKernelGGL((offsets_kernel<Src>), dim3(groups), dim3(kThreads), 0, cs_hip::raw(s), in, static_cast<uint64_t>(count_src),
-                       extents, offs, groups, static_cast<ScanHeader*>(workspace), result);
-    return cs_hip::check(hipGetLastError(), "offsets launch");
+    auto           go     = [&](void* ws) {
+        hipLaunchKernelGGL((offsets_kernel<Src>), dim3(groups), dim3(kThreads), 0, cs_hip::raw(s), in, static_cast<uint64_t>(count_src),
+                           extents, offs, groups, static_cast<ScanHeader*>(ws), result);
+        return cs_hip::check(hipGetLastError(), "offsets launch");
+    };
+    return workspace ? go(workspace) : with_owned_workspace(cs_hip_scan_workspace_bytes(count_src), s, go);
 }
 
 }  // namespace

@@ -204,3 +204,23 @@ __device__ inline T exclusive_scan(T x, Op op, T identity, T* total = nullptr) {
 }
 
 }  // namespace naga::cospan::block
+
+namespace naga::cospan::bounded {
+
+/// Polls `ready()` at most `limit` times with a short sleep between polls: true as soon as it holds, false when it never
+/// did. A limit of 0 polls nothing and gives up at once (the host's way to force the give-up path).
+template <class Ready>
+__device__ inline bool poll(Ready&& ready, std::uint32_t limit) {
+    for (std::uint32_t k = 0; k < limit; ++k) {
+        if (ready()) return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+
+/// ORs `bits` (CS_HIP_FAULT_*) into the device's fault word; a null word reports nowhere.
+__device__ inline void report(std::uint32_t* fault_word, std::uint32_t bits) {
+    if (fault_word != nullptr) __hip_atomic_fetch_or(fault_word, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace naga::cospan::bounded

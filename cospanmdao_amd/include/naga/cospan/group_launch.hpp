@@ -40,6 +40,24 @@ inline void hip_check(int status, const std::string& what) {
     throw std::runtime_error(text);
 }
 
+/// Reads and clears the device's fault word after completed work; throws std::runtime_error when a bounded device wait of
+/// that work gave up (its results are then not valid).
+inline void check_device_faults(const std::string& who) {
+    std::uint32_t bits = 0;
+    hip_check(cs_hip_take_fault(&bits), who);
+    if (bits == 0) return;
+    std::string what;
+    auto        name = [&](std::uint32_t bit, const char* text) {
+        if (bits & bit) what += (what.empty() ? "" : ", ") + std::string(text);
+    };
+    name(CS_HIP_FAULT_TICKET_WAIT, "a fused group's wait for an earlier group");
+    name(CS_HIP_FAULT_SCAN_LOOKBACK, "the device scan's look-back");
+    name(CS_HIP_FAULT_STATE, "ticket state or scan workspace not zeroed at launch");
+    name(CS_HIP_FAULT_UNIT_WAIT, "a work-queue kernel's wait for an earlier unit");
+    throw std::runtime_error("naga::cospan: " + who + ": a bounded device wait gave up (fault bits " + std::to_string(bits) + ": " +
+                             what + "); the results of that work are not valid");
+}
+
 /// What a launcher needs of one stage.
 struct StagePlan {
     std::uint64_t units[3] = {1, 1, 1};  ///< work-groups (group-range) or indices (index-range), leading 1s
@@ -310,9 +328,15 @@ private:
         }
 
         if (waits) geo.ticket_state = tickets_.ensure(ticket_words(nseg, geo.total_groups), s);
+        // where a wait that gives up reports, and how often a wait polls (both are argument bytes: a changed bound is a
+        // different launch, so a captured graph is not replayed with the old one)
+        hip_check(cs_hip_fault_word(&geo.fault), "fault word of '" + group_name + "'");
+        geo.spin_limit = cs_hip_spin_limit();
 
         static_assert(sizeof(GroupArgs<X...>) <= 4096, "cospan: a fused group's arguments exceed the 4 KiB kernel-argument limit");
         static_assert(std::is_trivially_copyable_v<GroupArgs<X...>>, "cospan: a fused group's arguments are copied as bytes");
+        static_assert(std::has_unique_object_representations_v<Geometry<N>>,
+                      "cospan: the geometry has padding bytes, which would differ between identical launches");
         rec.kernel = kernel();
         for (int d = 0; d < 3; ++d) rec.grid[d] = grid[d];
         rec.block[0] = wg[2];

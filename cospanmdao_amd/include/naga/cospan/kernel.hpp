@@ -33,6 +33,7 @@
 #include <type_traits>
 #include <utility>
 
+#include "cospan_hip.h"
 #include "naga/cospan/device.hpp"
 #include "naga/cospan/dtype.hpp"
 #include "naga/cospan/vocab.hpp"
@@ -82,11 +83,14 @@ struct SegmentInfo {
 template <int N>
 struct Geometry {
     std::uint32_t* ticket_state = nullptr;  ///< null: no ticket mode
+    std::uint32_t* fault        = nullptr;  ///< the device's fault word (cs_hip_fault_word): where a wait that gave up reports
     std::uint64_t  total_groups = 0;
     std::uint32_t  segments     = 0;
     std::uint32_t  split        = 0;
     std::uint32_t  seg_lo       = 0;  ///< the segments this launch runs: [seg_lo, seg_hi) (all of them but for a group
     std::uint32_t  seg_hi       = 0;  ///< whose global barriers are launch boundaries)
+    std::uint32_t  spin_limit   = CS_HIP_DEFAULT_SPIN_LIMIT;  ///< polls per wait before it gives up
+    std::uint32_t  unused       = 0;  ///< keeps the struct free of padding: a launch's signature compares its bytes
     StageGeometry  stage[N];
     SegmentInfo    seg[N];
 };
@@ -140,21 +144,32 @@ __device__ inline std::uint64_t segment_linear(const SegmentInfo& s, const std::
     return (c[0] * s.range[1] + c[1]) * s.range[2] + c[2];
 }
 
-/// Group number `lin` (a ticket or the flattened hardware id) -> segment and coordinate, segment-major then row-major
-/// (reference kernel.hpp:246-260).
+/// The segment that holds group number `lin` (a ticket, or a hardware id flattened along x) and the group's coordinate in
+/// it. Numbers run segment after segment (segment s owns [first_s, first_s + count_s), the segments back to back), and
+/// inside a segment over its rank-3 range with the last coordinate fastest. A number at or past `total` is no group.
 __device__ inline LogicalGroup locate_linear(const SegmentInfo* tbl, std::uint32_t segments, std::uint64_t lin, std::uint64_t total) {
-    LogicalGroup r;
-    if (lin >= total) return r;
-    std::uint32_t s = 0;
-    while (s + 1 < segments && lin >= tbl[s].first + tbl[s].count) ++s;
-    std::uint64_t k = lin - tbl[s].first;
-    r.segment       = s;
-    r.coord[2]      = k % tbl[s].range[2];
-    k /= tbl[s].range[2];
-    r.coord[1] = k % tbl[s].range[1];
-    r.coord[0] = k / tbl[s].range[1];
-    r.valid    = true;
-    return r;
+    LogicalGroup where;
+    if (lin >= total) return where;
+    // the owner is the segment before the first one that starts past `lin` (an empty segment starts where the next one
+    // does, so it never owns a number)
+    std::uint32_t seg = segments - 1;
+    for (std::uint32_t s = 1; s < segments; ++s)
+        if (lin < tbl[s].first) {
+            seg = s - 1;
+            break;
+        }
+    const SegmentInfo&  info  = tbl[seg];
+    const std::uint64_t rel   = lin - info.first;
+    const std::uint64_t plane = info.range[1] * info.range[2];
+    const std::uint64_t c0    = rel / plane;
+    const std::uint64_t rest  = rel - c0 * plane;
+    const std::uint64_t c1    = rest / info.range[2];
+    where.segment  = seg;
+    where.coord[0] = c0;
+    where.coord[1] = c1;
+    where.coord[2] = rest - c1 * info.range[2];
+    where.valid    = true;
+    return where;
 }
 
 /// Hardware-direct mapping (no tickets), per split strategy (reference kernel.hpp:266-288). Physical dimension 2 is
@@ -203,28 +218,32 @@ CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total)
 __device__ inline std::uint32_t poll_load(std::uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 /// Waits only ever target groups with earlier tickets, which have started and finish without waiting on later ones, so
-/// they end; the bound (about a second) only guarantees that corrupted ticket state cannot hang the device.
-inline constexpr std::uint32_t kSpinLimit = 1u << 24;
+/// they end. Each wait still polls at most `limit` times (Geometry::spin_limit, set from cs_hip_spin_limit at launch): one
+/// that gives up reports CS_HIP_FAULT_TICKET_WAIT to the fault word and lets its group run on, so corrupted ticket state
+/// can neither hang the device nor pass unnoticed.
 
 /// COLLECTIVE: wait for group `c` of segment `s` (reference kernel.hpp:215-224).
 __device__ inline void ticket_wait_group(std::uint32_t* ts, std::uint32_t segments, const SegmentInfo* tbl, std::uint32_t s,
-                                         const std::uint64_t* c) {
+                                         const std::uint64_t* c, std::uint32_t* fault, std::uint32_t limit) {
     if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
-        const std::uint64_t t = tbl[s].first + segment_linear(tbl[s], c);
-        for (std::uint32_t spin = 0; poll_load(&ts[kTicketHeader + segments + t]) == 0 && spin < kSpinLimit; ++spin)
-            __builtin_amdgcn_s_sleep(1);
+        std::uint32_t* const flag = &ts[kTicketHeader + segments + tbl[s].first + segment_linear(tbl[s], c)];
+        if (!bounded::poll([&] { return poll_load(flag) != 0; }, limit)) bounded::report(fault, CS_HIP_FAULT_TICKET_WAIT);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
 
 /// COLLECTIVE: wait until every group of segments 0 .. upto-1 has finished (reference kernel.hpp:227-236).
-__device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo* tbl, std::uint32_t upto) {
+__device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo* tbl, std::uint32_t upto, std::uint32_t* fault,
+                                            std::uint32_t limit) {
     if (threadIdx.x + threadIdx.y + threadIdx.z == 0) {
-        for (std::uint32_t i = 0; i < upto; ++i)
-            for (std::uint32_t spin = 0;
-                 static_cast<std::uint64_t>(poll_load(&ts[kTicketHeader + i])) != tbl[i].count && spin < kSpinLimit; ++spin)
-                __builtin_amdgcn_s_sleep(1);
+        for (std::uint32_t i = 0; i < upto; ++i) {
+            std::uint32_t* const done = &ts[kTicketHeader + i];
+            if (!bounded::poll([&] { return static_cast<std::uint64_t>(poll_load(done)) == tbl[i].count; }, limit)) {
+                bounded::report(fault, CS_HIP_FAULT_TICKET_WAIT);
+                break;
+            }
+        }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
@@ -349,12 +368,17 @@ struct GroupItem {
         if (ticket_state_ == nullptr) return;
         const id<3>         l = detail::lift_id<R>(g);
         const std::uint64_t c[3] = {l[0], l[1], l[2]};
-        detail::ticket_wait_group(ticket_state_, segments_, table_, static_cast<std::uint32_t>(s), c);
+        detail::ticket_wait_group(ticket_state_, segments_, table_, static_cast<std::uint32_t>(s), c, fault_, spin_limit_);
     }
     /// COLLECTIVE: wait until every group of every earlier segment has finished (what cs::global_barrier() does).
     __device__ void wait_earlier_segments() const {
-        if (ticket_state_ != nullptr) detail::ticket_wait_segments(ticket_state_, table_, segment_);
+        if (ticket_state_ != nullptr) detail::ticket_wait_segments(ticket_state_, table_, segment_, fault_, spin_limit_);
     }
+
+    /// The poll bound of this launch's waits; a stage with a bounded wait of its own polls at most this often and, when it
+    /// gives up, reports through report_fault (CS_HIP_FAULT_* bits, read by Model::wait).
+    __device__ std::uint32_t spin_limit() const { return spin_limit_; }
+    __device__ void          report_fault(std::uint32_t bits) const { bounded::report(fault_, bits); }
 
     __device__ Group<R>  get_group() const { return Group<R>{group_id_, group_range_}; }
     __device__ sub_group get_sub_group() const { return sub_group{}; }
@@ -367,6 +391,8 @@ struct GroupItem {
     const detail::SegmentInfo* table_;
     std::uint32_t*             ticket_state_;
     std::byte*                 base_;  ///< this stage's slice of the launch's LDS block
+    std::uint32_t*             fault_;
+    std::uint32_t              spin_limit_;
 };
 
 // -----------------------------------------------------------------------------
@@ -401,8 +427,29 @@ template <class A, class... As>
 struct nth_array<0, A, As...> { using type = A; };
 template <std::size_t I, class A, class... As>
 struct nth_array<I, A, As...> : nth_array<I - 1, As...> {};
+
+/// Where the arrays of a LocalMem slice sit: walking the declarations in order, each array starts at the first multiple
+/// of its element's alignment at or after the end of the one before. place() returns the end of the last array (the
+/// slice's size) and, when `starts` is given, writes each array's first byte there.
+template <class... As>
+struct LocalLayout {
+    static CS_HD std::size_t place(const range<3>*, std::size_t end, std::size_t*) { return end; }
+};
+template <class A, class... Rest>
+struct LocalLayout<A, Rest...> {
+    static CS_HD std::size_t place(const range<3>* shapes, std::size_t end, std::size_t* starts) {
+        using T                 = typename A::value_type;
+        const std::size_t first = align_up(end, alignof(T));
+        if (starts != nullptr) *starts++ = first;
+        return LocalLayout<Rest...>::place(shapes + 1, first + shapes->size() * sizeof(T), starts);
+    }
+};
 }  // namespace detail
 
+/// The local-memory arrays a group-range stage asks for (its fourth compute_forward parameter). It carries one shape per
+/// array; the arrays themselves live in the stage's slice of the launch's LDS block, laid out by detail::LocalLayout,
+/// and get() hands a work-item typed views of them in declaration order. Trivially copyable (it travels as a kernel
+/// argument).
 template <class... Arrays>
 struct LocalMem {
     static_assert(sizeof...(Arrays) >= 1 && sizeof...(Arrays) <= 6, "cospan: a LocalMem holds one to six arrays");
@@ -411,43 +458,37 @@ struct LocalMem {
     template <std::size_t I>
     using array_t = typename detail::nth_array<I, Arrays...>::type;
 
+    /// Every array empty.
     CS_HD LocalMem() {
-        for (std::size_t i = 0; i < count; ++i) shapes_[i] = range<3>{0, 0, 0};
+        for (range<3>& r : shapes_) r = range<3>{0, 0, 0};
     }
+    /// One shape per array, in declaration order.
     CS_HD explicit LocalMem(range<Arrays::rank>... shapes) : shapes_{detail::lift_range<Arrays::rank>(shapes)...} {}
 
     template <std::size_t I>
     CS_HD range<array_t<I>::rank> shape() const {
-        const range<3>& s = shapes_[I];
-        return detail::tail_range<array_t<I>::rank>(s[0], s[1], s[2]);
+        return detail::tail_range<array_t<I>::rank>(shapes_[I][0], shapes_[I][1], shapes_[I][2]);
     }
 
-    /// The bytes of the stage's slice: each array at a multiple of its element alignment.
-    CS_HD std::size_t bytes() const {
-        std::size_t cur = 0;
-        [&]<std::size_t... I>(std::index_sequence<I...>) {
-            ((cur = detail::align_up(cur, alignof(typename array_t<I>::value_type)) +
-                    shape<I>().size() * sizeof(typename array_t<I>::value_type)),
-             ...);
-        }(std::make_index_sequence<count>{});
-        return cur;
-    }
+    /// The size of the stage's slice in bytes.
+    CS_HD std::size_t bytes() const { return detail::LocalLayout<Arrays...>::place(shapes_, 0, nullptr); }
 
+    /// The views of this stage's arrays for the calling work-item (no synchronisation).
     template <int R>
     __device__ auto get(const GroupItem<R>& it) const {
-        std::byte* const base = it.base_;
-        return [&]<std::size_t... I>(std::index_sequence<I...>) {
-            std::size_t off[count] = {};
-            std::size_t cur        = 0;
-            ((cur = detail::align_up(cur, alignof(typename array_t<I>::value_type)), off[I] = cur,
-              cur += shape<I>().size() * sizeof(typename array_t<I>::value_type)),
-             ...);
-            return detail::LocalViews<typename array_t<I>::view_type...>{typename array_t<I>::view_type{
-                reinterpret_cast<typename array_t<I>::value_type*>(base + off[I]), shape<I>()}...};
-        }(std::make_index_sequence<count>{});
+        std::size_t starts[count];
+        detail::LocalLayout<Arrays...>::place(shapes_, 0, starts);
+        return views(it.base_, starts, std::make_index_sequence<count>{});
     }
 
     range<3> shapes_[count];
+
+private:
+    template <std::size_t... I>
+    __device__ auto views(std::byte* slice, const std::size_t* starts, std::index_sequence<I...>) const {
+        using Views = detail::LocalViews<typename array_t<I>::view_type...>;
+        return Views{typename array_t<I>::view_type{reinterpret_cast<typename array_t<I>::value_type*>(slice + starts[I]), shape<I>()}...};
+    }
 };
 
 // -----------------------------------------------------------------------------
@@ -783,14 +824,13 @@ struct GroupArgs {
 };
 
 struct ItemFixup {
-    template <int R>
-    __device__ static GroupItem<R> make(const StageGeometry& g, const SegmentInfo* tbl, std::uint32_t segments,
-                                        std::uint32_t* ticket_state, const std::uint64_t* coord, const std::uint64_t* own,
-                                        std::byte* lds) {
+    template <int R, int N>
+    __device__ static GroupItem<R> make(const StageGeometry& g, const Geometry<N>& geo, const std::uint64_t* coord,
+                                        const std::uint64_t* own, std::byte* lds) {
         const std::uint64_t* gr = own;
-        if (g.range_mode == static_cast<std::uint32_t>(RangeMode::whole)) gr = tbl[g.segment].range;
-        return GroupItem<R>{tail_id<R>(coord[0], coord[1], coord[2]), tail_range<R>(gr[0], gr[1], gr[2]), g.segment, segments,
-                            tbl, ticket_state, lds + g.local_offset};
+        if (g.range_mode == static_cast<std::uint32_t>(RangeMode::whole)) gr = geo.seg[g.segment].range;
+        return GroupItem<R>{tail_id<R>(coord[0], coord[1], coord[2]), tail_range<R>(gr[0], gr[1], gr[2]), g.segment, geo.segments,
+                            geo.seg, geo.ticket_state, lds + g.local_offset, geo.fault, geo.spin_limit};
     }
 };
 
@@ -805,7 +845,7 @@ __device__ inline void stage_run(const Geometry<N>& geo, const StageArgs<X>& a, 
             if (lg.coord[d] >= a.units[d]) return;
     constexpr int R = kernel_rank_v<X>;
     if constexpr (GroupRangeStage<X>) {
-        const GroupItem<R> gi = ItemFixup::make<R>(g, geo.seg, geo.segments, geo.ticket_state, lg.coord, a.units, lds);
+        const GroupItem<R> gi = ItemFixup::make<R>(g, geo, lg.coord, a.units, lds);
         if constexpr (has_local_mem_v<X>)
             X::compute_forward(gi, a.in, a.out, a.local);
         else
@@ -835,9 +875,13 @@ __global__ __launch_bounds__(MaxThreads) __attribute__((amdgpu_waves_per_eu(MinW
         __syncthreads();
         ticket = s_ticket;
         lg     = locate_linear(geo.seg, geo.segments, ticket, geo.total_groups);
-        if (!lg.valid) return;
+        if (!lg.valid) {
+            // a ticket past the last group: the state was not zero when the launch began (two launches sharing it)
+            if (local_linear_id() == 0) bounded::report(geo.fault, CS_HIP_FAULT_STATE);
+            return;
+        }
         if (geo.seg[lg.segment].boundary == static_cast<std::uint64_t>(Boundary::global_barrier))
-            ticket_wait_segments(ts, geo.seg, lg.segment);
+            ticket_wait_segments(ts, geo.seg, lg.segment, geo.fault, geo.spin_limit);
     } else {
         lg = locate_physical(geo.split, geo.seg, geo.seg_lo, geo.seg_hi, geo.total_groups);
         if (!lg.valid) return;

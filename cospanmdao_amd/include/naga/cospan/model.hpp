@@ -56,6 +56,7 @@ public:
         if (!data_.empty()) {
             detail::hip_check(cs_hip_memcpy_async(data_.data(), v.data, data_.size() * sizeof(value_type), ex_->stream(0)), "view sync");
             detail::hip_check(cs_hip_stream_synchronize(ex_->stream(0)), "view sync");
+            detail::check_device_faults("view sync");
         }
     }
     range<Rank>       shape() const { return shape_; }
@@ -103,6 +104,7 @@ public:
             detail::hip_check(cs_hip_memcpy_async(data_.data(), v.data, data_.size() * sizeof(value_type), s), "view sync");
             detail::hip_check(cs_hip_stream_synchronize(s), "view sync");
         }
+        detail::check_device_faults("view sync");
     }
     std::size_t                       size() const { return data_.size(); }
     const value_type*                 data() const { return data_.data(); }
@@ -596,6 +598,10 @@ private:
         std::uint32_t res[2] = {0, 0};
         detail::hip_check(cs_hip_memcpy_async(res, c.result, sizeof res, s), who);
         detail::hip_check(cs_hip_stream_synchronize(s), who);
+        if (res[1] & CS_HIP_FLAG_WAIT_TIMEOUT) {
+            detail::check_device_faults(who);  // takes the device's fault word with the scan's bits
+            throw std::runtime_error("naga::cospan: " + who + ": the column build's look-back gave up; its offsets are not valid");
+        }
         if ((res[1] & CS_HIP_FLAG_NEGATIVE) == 0 && (res[1] & CS_HIP_FLAG_EXCEEDS32) != 0)
             throw std::invalid_argument("naga::cospan: " + who + ": '" + h.source_path +
                                         "' holds a value or a total past 2^32 - 1; ragged totals are limited to 2^32 - 1 elements");

@@ -187,9 +187,11 @@ public:
             if (k != s && last_[k]) wait(s, last_[k]);
     }
 
+    /// Waits for every stream, then reports a bounded device wait of that work that gave up (std::runtime_error).
     void synchronize() {
         if (!ready_) return;
         for (int s = 0; s < kStreams; ++s) hip_check(cs_hip_stream_synchronize(streams_[s]), "synchronize");
+        check_device_faults("synchronize");
     }
 
     // ---- graphs -------------------------------------------------------------------------------------------------

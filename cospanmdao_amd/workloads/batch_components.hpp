@@ -419,28 +419,38 @@ public:
         const std::uint32_t tid      = static_cast<std::uint32_t>(it.get_local_id(0));
         __shared__ std::uint32_t s_unit;
 
-        // a work-group takes each unit at most once, so T + 1 fetches bound the loop whatever the state holds
+        // One leader-only block per iteration, between two barriers: it publishes the completion of the unit the group
+        // has just run (when a later unit waits on it: release fence, then the counter) and takes the next ticket. The only
+        // other leader-only block of the loop, a wait's poll, also sits between barriers, so no two of them can be merged
+        // across the back edge (such a merged pair once parked the leader in a nested loop while the other lanes re-read
+        // the previous unit). A work-group takes each unit at most once, so T + 1 fetches bound the loop whatever the
+        // state holds.
+        std::uint32_t* pending = nullptr;  // the completion counter of the unit just run (the same in every work-item)
         for (std::uint32_t fetches = 0; fetches <= T; ++fetches) {
-            // The barrier BEFORE the leader's fetch keeps the leader-only blocks of two iterations (the previous unit's
-            // completion and this fetch) apart: with nothing between them the compiler merges them across the back edge
-            // into a nested loop whose divergent exit lets the other lanes re-read the previous unit forever.
+            __syncthreads();  // every work-item's writes of the previous unit are issued, and every work-item has read s_unit
+            if (tid == 0) {
+                if (pending != nullptr) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    __hip_atomic_fetch_add(pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                s_unit = __hip_atomic_fetch_add(&st[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             __syncthreads();
-            if (tid == 0) s_unit = __hip_atomic_fetch_add(&st[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __syncthreads();
+            pending             = nullptr;
             const std::uint32_t u = s_unit;
             if (u >= T) break;
             if (u < T1) {
                 const std::uint32_t k = cols.find(3, u);
                 unit_pass1(a, cols.unit(sizes, 3, k, u), tid);
-                complete(&p1_done[k], tid);
+                pending = &p1_done[k];
             } else if (u < T1 + TR) {
                 const std::uint32_t k = cols.find(4, u - T1);
-                wait_for(&p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
+                wait_for(it, &p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
                 unit_reduce(a, cols.unit(sizes, 4, k, u - T1), tid);
-                complete(&red_done[k], tid);
+                pending = &red_done[k];
             } else {
                 const std::uint32_t k = cols.find(5, u - T1 - TR);
-                wait_for(&red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
+                wait_for(it, &red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
                 const Unit w = cols.unit(sizes, 5, k, u - T1 - TR);
                 if (RT == 4)
                     unit_pass2<4>(a, w, tid);
@@ -448,9 +458,15 @@ public:
                     unit_pass2<8>(a, w, tid);
             }
         }
-        // leaving: the last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
+        // leaving (a completion still pending only if the fetch bound ended the loop, which a zeroed state never does): the
+        // last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
+        __syncthreads();
         __shared__ std::uint32_t s_last;
         if (tid == 0) {
+            if (pending != nullptr) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_fetch_add(pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             const std::uint32_t f = __hip_atomic_fetch_add(&st[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             s_last                = f + 1 == static_cast<std::uint32_t>(it.get_group_range(0)) ? 1u : 0u;
         }
@@ -460,20 +476,11 @@ public:
     }
 
     // ---- the units ------------------------------------------------------------------------------------------------
-    /// After a unit: every work-item's writes, then the leader's release increment.
-    __device__ static void complete(std::uint32_t* counter, std::uint32_t tid) {
-        __syncthreads();
-        if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    /// Before a unit: the leader waits (acquire) for `target` completions of earlier units (bounded: a corrupted state
-    /// cannot hang the device), then the group.
-    __device__ static void wait_for(std::uint32_t* counter, std::uint32_t target, std::uint32_t tid) {
-        if (tid == 0)
-            for (std::uint32_t spin = 0; cs::detail::poll_load(counter) < target && spin < (1u << 24); ++spin)
-                __builtin_amdgcn_s_sleep(1);
+    /// Before a unit: the leader waits (acquire) for `target` completions of earlier units, at most the launch's spin limit
+    /// of polls (one that gives up reports CS_HIP_FAULT_UNIT_WAIT and the unit runs on stale data), then the group.
+    __device__ static void wait_for(const cs::GroupItem<1>& it, std::uint32_t* counter, std::uint32_t target, std::uint32_t tid) {
+        if (tid == 0 && !cs::bounded::poll([&] { return cs::detail::poll_load(counter) >= target; }, it.spin_limit()))
+            it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }

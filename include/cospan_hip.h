@@ -31,6 +31,10 @@ typedef struct cs_hip_graph_s*  cs_hip_graph;
 /* The last error message of this thread (empty when none). Returns the message length. */
 int cs_hip_last_error(char* buf, size_t len);
 
+/* The identity of the sources this library was built from (16 hex digits, cospanmdao_amd/source_hash.py); no reference
+ * counterpart (the reference compiles its headers into each user program). Tests compare it with the tree. */
+const char* cs_hip_build_id(void);
+
 /* ---- device ------------------------------------------------------------------------------------------------------
  * replaces: naga::cospan::detail::Runtime construction and device selection (cospanmdao/include/naga/cospan/runtime.hpp:33-50)
  */
@@ -91,12 +95,39 @@ int cs_hip_graph_destroy(cs_hip_graph g);
 /* 1 while the stream is capturing */
 int cs_hip_stream_capturing(cs_hip_stream s, int* capturing);
 
+/* ---- bounded device waits ------------------------------------------------------------------------------------------
+ * (no reference counterpart: the reference's ticket waits, cospanmdao/include/naga/cospan/kernel.hpp:197-236, spin
+ * without a bound.) Every spin-wait in device code of this build (a fused group's wait_group / global barrier in ticket
+ * mode, the scan's look-back, a work-queue kernel's waits) polls at most `limit` times. A wait that gives up ORs a bit
+ * below into the fault word of its device and the kernel runs on to its normal end; the results of that work are then
+ * not valid. A limit of 0 makes every wait give up without polling (a way for a test to force the condition from the
+ * host). The C++ layer's Model::wait and view synchronisations read and clear the word and throw std::runtime_error
+ * when it is set; callers of the C entry points read it with cs_hip_take_fault after their work has completed. The word
+ * is per device: the first reader after the faulting work takes it.
+ */
+#define CS_HIP_FAULT_TICKET_WAIT 1u   /* a fused group's wait for an earlier group or segment gave up */
+#define CS_HIP_FAULT_SCAN_LOOKBACK 2u /* the scan's look-back at a predecessor tile gave up */
+#define CS_HIP_FAULT_STATE 4u         /* a work-group found ticket state or a scan workspace that was not zeroed */
+#define CS_HIP_FAULT_UNIT_WAIT 8u     /* a work-queue kernel's wait for an earlier unit gave up */
+#define CS_HIP_DEFAULT_SPIN_LIMIT (1u << 24)
+/* The process-wide poll bound of device waits launched from now on (CS_HIP_DEFAULT_SPIN_LIMIT at start). */
+int      cs_hip_set_spin_limit(uint32_t limit);
+uint32_t cs_hip_spin_limit(void);
+/* The device address of the current device's fault word (what kernels OR their fault bits into). */
+int cs_hip_fault_word(uint32_t** device_word);
+/* Reads and clears the current device's fault word; *bits = 0 when no wait gave up. Call after the work completed. */
+int cs_hip_take_fault(uint32_t* bits);
+
 /* ---- device exclusive scan ---------------------------------------------------------------------------------------
  * replaces: naga::cospan::detail::exclusive_scan_u32 (cospanmdao/include/naga/cospan/device_scan.hpp:46-110):
  * out[0..n] = the exclusive scan of in[0..n) (n + 1 outputs, out[n] the total), running sums wrap mod 2^32.
  * One launch: a decoupled look-back scan (tickets in start order, wave-parallel look-back with ballot, DPP wave scans,
  * LDS staging). `workspace` must hold cs_hip_scan_workspace_bytes(n) bytes, zeroed once (cs_hip_scan_workspace_init);
- * every scan leaves it zeroed again. workspace == NULL uses a library-owned one (grown outside any capture).
+ * every scan leaves it zeroed again. Two scans that run at the same time need two workspaces. workspace == NULL uses a
+ * library-owned workspace of the current device and of stream `s` (one per stream, so scans on different streams never
+ * share one; it grows outside any capture, and an outgrown one is kept until the process ends because a captured graph
+ * may still name it). A graph that captured such a scan must be replayed on its capturing stream, or given a workspace.
+ * A look-back that gives up sets CS_HIP_FAULT_SCAN_LOOKBACK (cs_hip_take_fault).
  */
 size_t cs_hip_scan_workspace_bytes(size_t n);
 int    cs_hip_scan_workspace_init(void* workspace, size_t bytes, cs_hip_stream s);
@@ -107,11 +138,12 @@ int    cs_hip_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, vo
  * src_dtype: 0 i32, 1 u32, 2 i64, 3 u64. counts mode (offsets_mode 0): src is [count_src, k] row-major, column 0
  * drives extents[i] = max(v, 0) and offs = their exclusive scan (count_src + 1 entries). offsets mode: src holds
  * count_src = N + 1 offsets, extents[i] = offs[i + 1] - offs[i]. result[0] = the total, result[1] = the flag word
- * (1: negative / invalid, 2: past 32 bits or wrapped); both written by the last work-group to finish (device memory).
- * One launch in either mode.
+ * (1: negative / invalid, 2: past 32 bits or wrapped, 4: a bounded device wait gave up, so the offsets are not valid);
+ * both written by the last work-group to finish (device memory). One launch in either mode.
  */
 #define CS_HIP_FLAG_NEGATIVE 1u
 #define CS_HIP_FLAG_EXCEEDS32 2u
+#define CS_HIP_FLAG_WAIT_TIMEOUT 4u
 int cs_hip_build_column(int src_dtype, const void* src, size_t count_src, size_t k, int offsets_mode, uint32_t* extents,
                         uint32_t* offs, uint32_t* result, void* workspace, cs_hip_stream s);
 

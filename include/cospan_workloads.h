@@ -18,6 +18,10 @@
 extern "C" {
 #endif
 
+/* The identity of the sources this library was built from (16 hex digits, cospanmdao_amd/source_hash.py; no reference
+ * counterpart). Tests compare it with the tree. */
+const char* cs_workloads_build_id(void);
+
 /* ---- GEMVER --------------------------------------------------------------------------------------------------------
  * replaces: the Gemver System of examples/gemver-chain/systems.hpp:71-139 and its driver (main.cpp:154-234).
  * Styles (the first three are the reference's, systems.hpp:33-48):

@@ -309,7 +309,12 @@ void neighbor_case(std::size_t wg, std::size_t G, int forwards) {
 
 using namespace tfg;
 
+#ifndef CS_SOURCE_HASH
+#define CS_SOURCE_HASH "unknown"
+#endif
+
 int main() {
+    std::printf("build %s\n", CS_SOURCE_HASH);
     int cus = 0;
     cs_hip_device_info(0, nullptr, 0, &cus, nullptr, nullptr);
     const std::size_t big_wg = 1024, big_groups = 4 * static_cast<std::size_t>(std::max(cus, 1));

@@ -220,7 +220,12 @@ float side_got(cs::Model& m) {
 
 using namespace tpol;
 
+#ifndef CS_SOURCE_HASH
+#define CS_SOURCE_HASH "unknown"
+#endif
+
 int main() {
+    std::printf("build %s\n", CS_SOURCE_HASH);
 #ifdef CS_HOST_MOCK
     mock::set_fallback_kernel([](const void*, const std::uint32_t*, const std::uint32_t*) {});
     mock::register_kernel(reinterpret_cast<const void*>(&slow_increment), sizeof(SideArgs), [](const void* a, const std::uint32_t*, const std::uint32_t*) {

@@ -349,7 +349,12 @@ void run_counts(cs::Model& model, const Sizes& counts, bool fused) {
 
 using namespace trg;
 
+#ifndef CS_SOURCE_HASH
+#define CS_SOURCE_HASH "unknown"
+#endif
+
 int main() {
+    std::printf("build %s\n", CS_SOURCE_HASH);
 #ifdef CS_HOST_MOCK
     mock::set_fallback_kernel([](const void*, const std::uint32_t*, const std::uint32_t*) {});
 #endif

@@ -244,7 +244,12 @@ double rel_l2(std::span<const float> got, const std::vector<double>& want) {
 
 using namespace trg;
 
+#ifndef CS_SOURCE_HASH
+#define CS_SOURCE_HASH "unknown"
+#endif
+
 int main() {
+    std::printf("build %s\n", CS_SOURCE_HASH);
     const std::vector<std::vector<std::uint32_t>> batches{{3, 64, 200, 17, 1, 256, 90}, {255, 8, 0, 129}, {33}, {120, 120, 5, 300, 64, 2, 2, 77}};
     for (bool fused : {false, true}) {
         std::printf("a batch of GEMVERs through ragged ports and counts_from hooks%s\n", fused ? ", the two passes fused" : "");

@@ -11,6 +11,8 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "mock_hip.hpp"
@@ -172,6 +174,14 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
           (unsigned long long)g_gemver_launches, st[0], st[1]);
     if (!zero) return;
 
+    // the launch carries the device's fault word and the current poll bound; a bound of 0 makes every wait give up
+    const auto& geo   = static_cast<const GAGem*>(bytes)->geo;
+    std::uint32_t* fw = nullptr;
+    cs_hip_fault_word(&fw);
+    CHECK(geo.fault == fw, "gemver: the launch does not carry the fault word");
+    CHECK(geo.spin_limit == cs_hip_spin_limit(), "gemver: the launch carries spin limit %u, the process %u", geo.spin_limit,
+          cs_hip_spin_limit());
+
     Emu e = emu_common(a.in, "gemver");
     e.A = cs::raw::data(a.in.A), e.U1 = cs::raw::data(a.in.u1), e.V1 = cs::raw::data(a.in.v1), e.U2 = cs::raw::data(a.in.u2);
     e.V2 = cs::raw::data(a.in.v2), e.Y = cs::raw::data(a.in.y), e.Z = cs::raw::data(a.in.z);
@@ -182,6 +192,7 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
     CHECK(a.in.A.size() >= 4ull * e.total(1), "gemver: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
     CHECK(a.out.P.size() >= e.total(2), "gemver: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
     CHECK(a.out.x.size() >= e.total(0) && a.in.u1.size() >= e.total(0), "gemver: vectors shorter than the scan's total");
+    if (geo.spin_limit == 0 && TR + T2 > 0) *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // the waits give up without polling
     std::vector<std::uint32_t> p1(K, 0), red(K, 0);
     for (std::uint32_t u = 0; u < T1 + TR + T2; ++u) {
         if (u < T1) {
@@ -385,6 +396,49 @@ static void run_form(bool phased, cs::ReplayMode mode = cs::ReplayMode::graph) {
     }
 }
 
+/// Bounded waits on the host side: a launch carries the fault word and the current spin limit; with a limit of 0 the
+/// work-queue kernel's waits give up, Model::wait throws std::runtime_error naming them and clears the word, and a
+/// captured graph is not replayed with a stale limit (the limit is part of the launch's argument bytes).
+static void run_fault_reporting() {
+    const std::uint32_t        count = 6, seed = 2;
+    std::vector<std::uint32_t> sizes(count);
+    or_ragged_sizes(seed, count, 16, 300, sizes.data());
+    g_sizes = sizes.data();
+    g_K     = count;
+    auto layout = std::make_shared<batch::Layout>(sizes, batch::BatchedGemver::kMinR);
+    auto knobs  = std::make_shared<batch::Knobs>();
+    std::vector<std::uint32_t> params{32, 4, 3};
+    knobs->groups = 3;
+    batch::count_units(sizes.data(), count, 32, 4, knobs->units);
+    cs::Model m = cs::create_model<batch::BatchedGemver>(layout, knobs, params, 1.5f, 1.2f, seed, host_a_init(seed * 16 + 1), false);
+    m.setup();
+    for (int i = 0; i < 3; ++i) m.run();  // eager, captured, replayed
+    m.wait();
+    cs_hip_set_spin_limit(0);
+    m.run();
+    bool        threw = false;
+    std::string msg;
+    try {
+        m.wait();
+    } catch (const std::runtime_error& e) {
+        threw = true;
+        msg   = e.what();
+    }
+    CHECK(threw && msg.find("gave up") != std::string::npos && msg.find("work-queue") != std::string::npos,
+          "faults: a wait that gave up was not reported (threw %d: %s)", threw ? 1 : 0, msg.c_str());
+    bool again = false;
+    try {
+        m.wait();
+    } catch (const std::exception&) {
+        again = true;
+    }
+    CHECK(!again, "faults: the fault word was not cleared by the wait that reported it");
+    cs_hip_set_spin_limit(CS_HIP_DEFAULT_SPIN_LIMIT);
+    for (int i = 0; i < 3; ++i) m.run();
+    check_outputs(m, sizes, 32, seed, 1.5f, 1.2f, "faults: runs after the limit is restored");
+    std::printf("faults: a work-queue wait that gave up is reported by Model::wait\n");
+}
+
 int main() {
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::index_kernel<batch::BatchPlan>), sizeof(GAPlan), emulate_plan);
     mock::register_kernel(reinterpret_cast<const void*>(&cs::detail::group_kernel<batch::kWg, 1, batch::RaggedGemver>), sizeof(GAGem),
@@ -398,6 +452,7 @@ int main() {
     run_form(false);
     run_form(true);
     run_form(true, cs::ReplayMode::direct);
+    run_fault_reporting();
 
     const std::string f = mock::failure();
     CHECK(f.empty(), "mock: %s", f.c_str());

@@ -402,6 +402,31 @@ int cs_hip_graph_destroy(cs_hip_graph g) {
     delete reinterpret_cast<Graph*>(g);
     return CS_HIP_OK;
 }
+const char* cs_hip_build_id(void) { return "mock"; }
+
+// the fault word and spin limit: a host word that emulated kernels OR their bits into, taken after the queues drain
+namespace {
+std::uint32_t g_fault_word = 0;
+std::uint32_t g_spin_limit = CS_HIP_DEFAULT_SPIN_LIMIT;
+}  // namespace
+int cs_hip_set_spin_limit(uint32_t limit) {
+    g_spin_limit = limit;
+    return CS_HIP_OK;
+}
+uint32_t cs_hip_spin_limit(void) { return g_spin_limit; }
+int      cs_hip_fault_word(uint32_t** p) {
+    if (!p) return fail(CS_HIP_ERR_INVALID, "cs_hip_fault_word: null");
+    *p = &g_fault_word;
+    return CS_HIP_OK;
+}
+int cs_hip_take_fault(uint32_t* bits) {
+    if (!bits) return fail(CS_HIP_ERR_INVALID, "cs_hip_take_fault: null");
+    drain();
+    *bits        = g_fault_word;
+    g_fault_word = 0;
+    return CS_HIP_OK;
+}
+
 int cs_hip_stream_capturing(cs_hip_stream, int* c) {
     *c = S().capturing ? 1 : 0;
     return CS_HIP_OK;

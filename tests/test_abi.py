@@ -15,7 +15,7 @@ HEADERS = {"cospan_hip.h": "libcospan_hip.so", "cospan_workloads.h": "libcospan_
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|void)\s+(cs_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|void|uint32_t|const\s+char\s*\*)\s*(cs_\w+)\s*\(", text, flags=re.M)))
 
 
 def exported(lib):
@@ -55,3 +55,14 @@ def test_device_count_without_a_gpu_is_not_an_error():
     n = C.c_int(-1)
     check(hip.cs_hip_device_count(C.byref(n)))
     assert n.value >= 0
+
+
+def test_libraries_are_built_from_this_tree():
+    """Each library carries the hash of the sources it was built from; a library built from other sources (a stale build)
+    fails here before any of its kernels could run."""
+    from cospanmdao_amd import hip, workloads
+    from cospanmdao_amd.source_hash import source_hash
+
+    want = source_hash(ROOT)
+    assert hip.cs_hip_build_id().decode() == want, "libcospan_hip.so is stale: run __graft_entry__.build()"
+    assert workloads.cs_workloads_build_id().decode() == want, "libcospan_workloads.so is stale: run __graft_entry__.build()"

@@ -6,49 +6,43 @@ import subprocess
 
 import pytest
 
+from cospanmdao_amd.source_hash import ROOT, source_hash
+
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def test_fused_groups_program():
-    exe = os.path.join(HERE, "framework", "_build", "fused_groups")
-    assert os.path.exists(exe), "tests/framework/_build/fused_groups is not built (make -C tests/framework)"
+def run_program(name):
+    """Runs tests/framework/_build/<name> after checking that it was built from the sources in this tree."""
+    exe = os.path.join(HERE, "framework", "_build", name)
+    assert os.path.exists(exe), f"tests/framework/_build/{name} is not built (make -C tests/framework)"
     print("running", exe, flush=True)
     r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
     print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
+    want = source_hash(ROOT, [os.path.join(HERE, "framework", name + ".cpp")])
+    assert r.stdout.startswith(f"build {want}\n"), f"{name} is stale (built from other sources than {want}): make -C tests/framework"
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+
+
+def test_fused_groups_program():
+    run_program("fused_groups")
 
 
 def test_ragged_program():
     """Ragged ports through cs::Model on the device: tests/framework/ragged.cpp (counts_from / offsets_from hooks, declared
     counts, dense sources read as rows, a ragged port in a fused group, the error paths), every value checked."""
-    exe = os.path.join(HERE, "framework", "_build", "ragged")
-    assert os.path.exists(exe), "tests/framework/_build/ragged is not built (make -C tests/framework)"
-    print("running", exe, flush=True)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
-    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
-    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+    run_program("ragged")
 
 
 def test_policies_program():
     """Run policies, rearm and depends_on through cs::Model on the device: tests/framework/policies.cpp (the reference's
     test_r04.cpp policy and dearm cases and test_system.cpp's depends_on side channel, over captured and replayed runs)."""
-    exe = os.path.join(HERE, "framework", "_build", "policies")
-    assert os.path.exists(exe), "tests/framework/_build/policies is not built (make -C tests/framework)"
-    print("running", exe, flush=True)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
-    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
-    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+    run_program("policies")
 
 
 def test_ragged_gemver_program():
     """Batched GEMVERs of different sizes written only with ragged ports and counts_from hooks (tests/framework/
     ragged_gemver.cpp): bare and fused passes, sizes and instance counts changing between forwards, every instance against
     FP64 (rel-L2 <= 1e-5)."""
-    exe = os.path.join(HERE, "framework", "_build", "ragged_gemver")
-    assert os.path.exists(exe), "tests/framework/_build/ragged_gemver is not built (make -C tests/framework)"
-    print("running", exe, flush=True)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
-    print(r.stdout[-4000:], r.stderr[-2000:], flush=True)
-    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-3000:]
+    run_program("ragged_gemver")

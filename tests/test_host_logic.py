@@ -31,6 +31,7 @@ def test_batched_gemver_host_logic():
     build()
     out = run("batch_host")
     assert "capture + replays" in out
+    assert "a work-queue wait that gave up is reported by Model::wait" in out
 
 
 def test_chain_host_logic():
