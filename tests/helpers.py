@@ -43,3 +43,23 @@ class DeviceArray:
 
 def init_device():
     check(hip.cs_hip_init(0))
+
+
+def record_errors(key, record):
+    """Merges `record` (per-output errors of one full-size parity case) under `key` into $CS_PARITY_OUT/parity_errors.json
+    when CS_PARITY_OUT is set; the committed summary profiles/parity_errors_full_size.md is made from that file."""
+    import json
+    import os
+
+    out = os.environ.get("CS_PARITY_OUT")
+    if not out:
+        return
+    os.makedirs(out, exist_ok=True)
+    path = os.path.join(out, "parity_errors.json")
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[key] = record
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
