@@ -88,6 +88,8 @@ def ref():
         R.ref_chain.argtypes = [_S] + [_P] * 13
         R.ref_relative_l2.restype = C.c_double
         R.ref_relative_l2.argtypes = [_P, _P, _S]
+        R.ref_relative_l2_f64.restype = C.c_double
+        R.ref_relative_l2_f64.argtypes = [_P, _P, _S]
         _ref = R
     return _ref
 

@@ -19,6 +19,11 @@
  *   bf16 / e4m3           exhaustive round trips over every code, round-to-nearest-even tie cases
  *   NS residual           uniform fields give exactly 0; a single Fourier mode matches the analytic derivatives to
  *                         O(h^2)
+ *   or_ns_fields          its Fourier transform: all energy on integer wavenumbers in [-4, 4]^2, no mean, at most 8
+ *                         conjugate pairs (7.3 on average), the 1/|k|^2 amplitude law with unit-variance draws
+ *                         (Parseval), uniform phases
+ *   or_relative_l2_*      against the reference's relative_l2 (chain_data.hpp:147-161) through oracle/_ref, bit for
+ *                         bit, with zero vectors, NaN and infinity; closed forms
  *   or_ragged_sizes       bounds and the log-uniform spread only: parity unpinned (no reference counterpart)
  */
 #include "oracle.h"

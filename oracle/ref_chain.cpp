@@ -41,4 +41,8 @@ double ref_relative_l2(const float* got, const double* want, std::uint64_t n) {
     return gemver_chain::relative_l2(std::vector<float>(got, got + n), std::vector<double>(want, want + n));
 }
 
+double ref_relative_l2_f64(const double* got, const double* want, std::uint64_t n) {
+    return gemver_chain::relative_l2(std::vector<double>(got, got + n), std::vector<double>(want, want + n));
+}
+
 }  // extern "C"

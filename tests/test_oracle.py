@@ -433,3 +433,95 @@ def test_chain_f32_against_fp64(onepass, R):
     for k in O.CHAIN_OUT:
         assert O.rel_l2(got[k], want[k]) < 1e-5, k
     assert O.rel_l2(got["B2"].ravel(), want["B2"].ravel()) < 1e-6
+
+
+# ---- the error metric against the reference's own relative_l2 (chain_data.hpp:147-161) ----------------------------------
+
+def _rel_cases():
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in (1, 2, 7, 1000, 4097):
+        want = rng.standard_normal(n)
+        got = (want + rng.standard_normal(n) * 10.0 ** rng.integers(-9, 0)).astype(np.float32)
+        cases.append((f"random n={n}", got, want))
+    cases.append(("both zero", np.zeros(5, np.float32), np.zeros(5)))
+    cases.append(("want zero, got not", np.array([0, 1e-30, 0], np.float32), np.zeros(3)))
+    cases.append(("NaN in got", np.array([1, np.nan, 3], np.float32), np.array([1.0, 2.0, 3.0])))
+    cases.append(("inf in got", np.array([1, np.inf, 3], np.float32), np.array([1.0, 2.0, 3.0])))
+    cases.append(("exact", np.array([1.5, -2.25, 1e-20], np.float32), np.array([1.5, -2.25, float(np.float32(1e-20))])))
+    cases.append(("got zero", np.zeros(4, np.float32), np.array([3.0, -4.0, 0.0, 12.0])))
+    cases.append(("empty", np.zeros(0, np.float32), np.zeros(0)))
+    cases.append(("huge and tiny", np.array([1e30, 1e-30], np.float32), np.array([1e30, 2e-30])))
+    return cases
+
+
+@pytest.mark.skipif(O.ref() is None, reason="oracle/_ref not built (the reference checkout is absent)")
+@pytest.mark.parametrize("name,got,want", _rel_cases(), ids=[c[0] for c in _rel_cases()])
+def test_relative_l2_matches_reference(name, got, want):
+    """or_relative_l2_f32 / _f64 give the reference's relative_l2 bit for bit (the same value, or both infinite) on
+    random vectors, two zero vectors (0), a zero want (infinity), NaN or infinity in got, empty vectors."""
+    L, R = O.lib(), O.ref()
+    got = np.ascontiguousarray(got, np.float32)
+    want = np.ascontiguousarray(want, np.float64)
+    g64 = got.astype(np.float64)
+    ours = L.or_relative_l2_f32(O.ptr(got), O.ptr(want), got.size)
+    ref = R.ref_relative_l2(O.ptr(got), O.ptr(want), got.size)
+    ours64 = L.or_relative_l2_f64(O.ptr(g64), O.ptr(want), got.size)
+    ref64 = R.ref_relative_l2_f64(O.ptr(g64), O.ptr(want), got.size)
+    for a, b in ((ours, ref), (ours64, ref64)):
+        assert (math.isinf(a) and math.isinf(b)) or np.float64(a).tobytes() == np.float64(b).tobytes(), (name, a, b)
+
+
+def test_relative_l2_closed_forms():
+    """Values the definition fixes: got = (1 + e) want gives e; got = 0 gives 1; one wrong entry among equal ones gives
+    its share; a NaN anywhere gives infinity (the reference's rule); the f32 and f64 forms agree on f32-exact data."""
+    L = O.lib()
+    want = np.array([3.0, 4.0])  # ||want|| = 5
+    for got, expect in (([3.0, 4.0], 0.0), ([0.0, 0.0], 1.0), ([3.0, 4.5], 0.1), ([6.0, 8.0], 1.0), ([-3.0, -4.0], 2.0)):
+        g = np.array(got, np.float32)
+        assert L.or_relative_l2_f32(O.ptr(g), O.ptr(want), 2) == pytest.approx(expect, abs=1e-15)
+        g64 = np.array(got)
+        assert L.or_relative_l2_f64(O.ptr(g64), O.ptr(want), 2) == pytest.approx(expect, abs=1e-15)
+    assert O.rel_linf(np.array([3.0, 4.5], np.float32), want) == pytest.approx(0.125)
+    assert O.rel_linf(np.array([3.0, 4.0], np.float32), want) == 0.0
+    assert math.isinf(O.rel_linf(np.array([np.nan, 4.0], np.float32), want))
+    assert math.isinf(O.rel_linf(np.array([1.0], np.float32), np.zeros(1)))
+
+
+# ---- the NS fields: what the Fourier transform fixes ----------------------------------------------------------------------
+
+def test_ns_fields_spectrum():
+    """u, v, p = sum of 8 modes a_m / |k_m|^2 cos(2 pi (kx x + ky y) + phi_m), integer k in [-4, 4]^2 minus 0, a_m ~ N(0, 1),
+    phi_m ~ U[0, 2 pi). Over 50 seeds x 3 fields the FFT of each field must have: all its energy on integer wavenumbers
+    inside the box (a non-integer wavenumber or a non-periodic field leaks across every bin), no mean, at most 8
+    conjugate pairs and about 7.3 on average (8 draws from 40 pairs; one mode in place of eight gives 1), sum over pairs of
+    |c_k|^2 |k|^4 / 8 with mean 1 (the 1/|k|^2 law with unit-variance amplitudes; 1/|k| gives about 13, 1/|k|^3 about
+    0.2), and phases spread uniformly."""
+    nx, ny = 64, 48
+    pairs_per_field, law, phases = [], [], []
+    for seed in range(50):
+        fields = [np.empty(nx * ny, np.float32) for _ in range(3)]
+        O.lib().or_ns_fields(seed, nx, ny, *[O.ptr(f) for f in fields])
+        for f in fields:
+            F = np.fft.fft2(f.reshape(ny, nx).astype(np.float64)) / (nx * ny)
+            total = np.sum(np.abs(F) ** 2)
+            ky = np.fft.fftfreq(ny, 1.0 / ny).round().astype(int)[:, None] * np.ones((1, nx), int)
+            kx = np.fft.fftfreq(nx, 1.0 / nx).round().astype(int)[None, :] * np.ones((ny, 1), int)
+            inside = (np.abs(kx) <= 4) & (np.abs(ky) <= 4)
+            assert np.sum(np.abs(F[~inside]) ** 2) <= 1e-12 * total, "energy outside the integer wavenumber box"
+            assert abs(F[0, 0]) ** 2 <= 1e-12 * total, "the field has a mean"
+            live = np.abs(F) ** 2 > 1e-10 * total
+            # one representative per conjugate pair: ky > 0, or ky == 0 and kx > 0
+            rep = live & ((ky > 0) | ((ky == 0) & (kx > 0)))
+            pairs = int(np.sum(rep))
+            assert 1 <= pairs <= 8, pairs
+            assert int(np.sum(live)) == 2 * pairs, "bins must come in conjugate pairs (a real field)"
+            pairs_per_field.append(pairs)
+            c = 2.0 * F[rep]  # a cos(theta + phi) puts a e^{i phi} / 2 on bin +k
+            k2 = (kx[rep] ** 2 + ky[rep] ** 2).astype(np.float64)
+            law.append(float(np.sum(np.abs(c) ** 2 * k2 ** 2)) / 8.0)
+            phases.extend(list(c / np.abs(c)))
+    mean_pairs = float(np.mean(pairs_per_field))
+    assert 6.8 <= mean_pairs <= 7.8, mean_pairs
+    assert 0.8 <= float(np.mean(law)) <= 1.2, float(np.mean(law))
+    assert abs(np.mean(phases)) < 0.1, abs(np.mean(phases))
