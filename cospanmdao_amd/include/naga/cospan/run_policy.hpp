@@ -16,7 +16,6 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
-#include <tuple>
 #include <type_traits>
 
 namespace naga::cospan {
@@ -36,38 +35,59 @@ inline constexpr bool is_property_v = std::is_same_v<std::remove_cvref_t<P>, pro
                                       std::is_same_v<std::remove_cvref_t<P>, property::do_once>;
 }  // namespace detail
 
-/// Any subset of the properties, each at most once (reference run_policy.hpp property_list).
+/// The run-policy properties given to add_subsystem / set_run_policy: a set of flags (which properties were given) plus
+/// run_n's count. Converts implicitly from one or more properties, so `{cs::property::transient{}}` reads as the
+/// reference's property_list does; a property given twice, run_n{0}, or run_n with do_once is an std::invalid_argument.
 class property_list {
 public:
     property_list() = default;
     template <class... Ps>
         requires(sizeof...(Ps) >= 1 && (detail::is_property_v<Ps> && ...))
-    property_list(Ps... ps) {  // NOLINT: implicit, as a SYCL property list
-        (set(std::move(ps)), ...);
-        if (has_property<property::run_n>() && get_property<property::run_n>().n == 0)
-            throw std::invalid_argument("naga::cospan: run_n needs n >= 1");
-        if (has_property<property::run_n>() && has_property<property::do_once>())
+    property_list(Ps... ps) {  // implicit on purpose: the reference API takes braced property lists
+        (add(ps), ...);
+        if ((given_ & kRunN) && turns_ == 0) throw std::invalid_argument("naga::cospan: run_n needs n >= 1");
+        if ((given_ & kRunN) && (given_ & kOnce))
             throw std::invalid_argument("naga::cospan: run_n and do_once are two run policies; give one");
     }
+
     template <class P>
     bool has_property() const {
-        return std::get<std::optional<P>>(props_).has_value();
+        return (given_ & flag<P>()) != 0;
     }
+    /// The property as given (run_n with its count).
     template <class P>
-    const P& get_property() const {
-        const auto& o = std::get<std::optional<P>>(props_);
-        if (!o) throw std::invalid_argument("naga::cospan: the property_list does not hold that property");
-        return *o;
+    P get_property() const {
+        if (!has_property<P>()) throw std::invalid_argument("naga::cospan: the property_list does not hold that property");
+        if constexpr (std::is_same_v<P, property::run_n>)
+            return property::run_n{turns_};
+        else
+            return P{};
     }
 
 private:
+    static constexpr unsigned kTransient = 1u, kRunN = 2u, kOnce = 4u;
     template <class P>
-    void set(P p) {
-        auto& o = std::get<std::optional<P>>(props_);
-        if (o) throw std::invalid_argument("naga::cospan: a property was given twice");
-        o = std::move(p);
+    static constexpr unsigned flag() {
+        if constexpr (std::is_same_v<P, property::transient>)
+            return kTransient;
+        else if constexpr (std::is_same_v<P, property::run_n>)
+            return kRunN;
+        else
+            return kOnce;
     }
-    std::tuple<std::optional<property::transient>, std::optional<property::run_n>, std::optional<property::do_once>> props_;
+    void mark(unsigned f) {
+        if (given_ & f) throw std::invalid_argument("naga::cospan: a property was given twice");
+        given_ |= f;
+    }
+    void add(property::transient) { mark(kTransient); }
+    void add(property::do_once) { mark(kOnce); }
+    void add(property::run_n r) {
+        mark(kRunN);
+        turns_ = r.n;
+    }
+
+    unsigned    given_ = 0;
+    std::size_t turns_ = 0;
 };
 
 namespace detail {
@@ -79,17 +99,17 @@ struct RunPolicy {
     void        rearm() { left = n; }
 };
 
-/// The policy a property_list asks for, or none. `where` prefixes the errors.
+/// The policy `props` asks for (none when it names no property); `where` names the subsystem in the error.
 inline std::optional<RunPolicy> make_run_policy(const property_list& props, const std::string& where) {
-    const bool tr    = props.has_property<property::transient>();
-    const bool has_n = props.has_property<property::run_n>();
-    const bool once  = props.has_property<property::do_once>();
-    if (tr && (has_n || once)) throw std::invalid_argument("naga::cospan: " + where + ": transient cannot be combined with run_n or do_once");
-    if (tr) return RunPolicy{RunPolicy::Kind::transient, 0, 0};
-    if (has_n || once) {
-        const std::size_t n = has_n ? props.get_property<property::run_n>().n : 1;
-        return RunPolicy{RunPolicy::Kind::counted, n, n};
-    }
+    // do_once is run_n{1}; a count of 0 means no counted policy was asked for (run_n{0} never gets this far)
+    const std::size_t turns     = props.has_property<property::run_n>()  ? props.get_property<property::run_n>().n
+                                  : props.has_property<property::do_once>() ? 1
+                                                                            : 0;
+    const bool        transient = props.has_property<property::transient>();
+    if (transient && turns != 0)
+        throw std::invalid_argument("naga::cospan: " + where + ": transient cannot be combined with run_n or do_once");
+    if (transient) return RunPolicy{RunPolicy::Kind::transient, 0, 0};
+    if (turns != 0) return RunPolicy{RunPolicy::Kind::counted, turns, turns};
     return std::nullopt;
 }
 

@@ -4,28 +4,30 @@
 // cospanmdao_amd/workloads/chain_systems.hpp — the GEMVER -> BiCGK -> GEMVER chain as Systems, in the reference's styles
 // =============================================================================
 //
-// The wiring of examples/gemver-chain/systems.hpp:60-346, over this build's components (gemver_components.hpp,
-// chain_components.hpp). Every root promotes the same names, so a driver reads one set of paths whatever the style:
-//   inputs   A (G1's A0), u1a v1a u2a v2a ya za (G1's vectors), u1b v1b u2b v2b (G2's copies of u1 .. v2)
-//   outputs  B1 x1 w1 (G1), q s (BiCGK), B2 x2 w2 (G2)
-// GEMVER here is the reference's chain GEMVER: one `scale` tied into cred and gemv, alpha = beta = 1/sqrt(n)
-// (systems.hpp:79, chain_data.hpp:13). BiCGK: q = B p, s = B^T r with p = x1, r = w1 (scale 1); G2: A = B1, y = q, z = s.
+// What the chain computes: G1 = GEMVER(A; y, z) giving B1, x1, w1; BiCGK on B1 with p = x1, r = w1 giving q = B1 p and
+// s = B1^T r; G2 = GEMVER(B1; y = q, z = s) giving B2, x2, w2. Inside each GEMVER alpha = beta = 1/sqrt(n), held by one
+// `scale` input that the column reduction and the row GEMV share through a tie (the reference's convention,
+// examples/gemver-chain/systems.hpp:79 and chain_data.hpp:13); BiCGK runs at scale 1.
 //
-// Styles (systems.hpp:38-53):
-//   unfused   every component its own launch
-//   parallel  as unfused, but BiCGK's gemv_q and gtp_s are a system_from group (independent: they overlap on two streams)
-//   fused     GEMVER: fuse_components(segment<range_mode::whole>(r2u, gtp)), cred, gemv; BiCGK: fuse_components(gemv_q,
-//             gtp_s) (one launch, a segment each), cred_s
-//   onepass   fused GEMVERs, BiCGK from one pass over B (BicgkOnePass, RowReduce, ColReduce<false>)
-//   naive     three fused groups with global barriers: G1 and G2 (r2u+gtp | cred | gemv), BiCGK (onepass | rowred, cred_s)
-//   crafted   three fused groups over the local-memory components: g1 (r2u_local+gtp_local | cred | gemv), mid
-//             (bicgk_tile+r2u_from_local | rowred, cred_s: BiCGK and G2's rank-2 update share ONE read of B1), g2 (gtp |
-//             cred | gemv); its strips are kRt rows
-// unfused, parallel and fused sum in the same order (bit-identical results); onepass and naive share BiCGK's order; crafted
-// equals onepass at R = kRt.
+// Port names at the root, the same in every style so that cs_chain_* reads one set of paths:
+//   in   A; u1a v1a u2a v2a ya za (G1); u1b v1b u2b v2b (G2's own copies of the rank-2 vectors)
+//   out  B1 x1 w1; q s; B2 x2 w2
 //
-// Inputs: the reference's hash generator on the device (chain_data.hpp:34-72, seeds A = 1 .. z = 7); G2's u1b .. v2b are
-// the same vectors as G1's (main.cpp:85-93).
+// Launches per chain and what shares a launch (the reference's six styles, systems.hpp:38-53; components from
+// gemver_components.hpp and chain_components.hpp):
+//   unfused   11  one launch per component
+//   parallel  11  the same launches; BiCGK's two products are independent leaves, so the executor puts them on two streams
+//   fused      8  per GEMVER one launch for rank-2 update + column partials (one segment over the whole range), then the
+//                 column reduction and the row GEMV; BiCGK's two products in one launch (a segment each)
+//   onepass    9  fused GEMVERs; BiCGK reads B1 once (BicgkOnePass) and reduces its row and column partials
+//   naive      8  three fused groups whose global barriers become launch boundaries: G1, BiCGK, G2
+//   crafted    8  three fused groups over the LDS-tile components; the middle one lets BiCGK and G2's rank-2 update share
+//                 one read of B1 (strips of kRt rows)
+// Results: unfused, parallel and fused add in one order, onepass and naive in another, so each class agrees bit for bit;
+// crafted equals onepass at R = kRt.
+//
+// Inputs come from the reference's counter hash on the device (chain_data.hpp:34-72, seeds A = 1 .. z = 7); G2's u1b ..
+// v2b hold the same values as G1's vectors (main.cpp:85-93).
 
 #include <cmath>
 #include <cstdint>
