@@ -66,6 +66,7 @@ HIP_API = [
     ("cs_hip_scan_workspace_bytes", _S, [_S]),
     ("cs_hip_scan_workspace_init", _I, [_P, _S, _P]),
     ("cs_hip_exclusive_scan_u32", _I, [_P, _P, _S, _P, _P]),
+    ("cs_hip_exclusive_scan_u32_static", _I, [_P, _P, _S, _P]),
     ("cs_hip_build_column", _I, [_I, _P, _S, _S, _I, _P, _P, _P, _P, _P]),
     ("cs_hip_fill_hash_f32", _I, [_U32, _P, _S, _S, _S, _P]),
     ("cs_hip_quantize_bf16", _I, [_P, _P, _S, _S, _S, _S, _P]),

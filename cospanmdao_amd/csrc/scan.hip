@@ -299,6 +299,40 @@ WaitBound wait_bound() {
 
 uint32_t tiles_for(size_t n) { return static_cast<uint32_t>(n == 0 ? 1 : (n + kTile - 1) / kTile); }
 
+// ---- the scan that allocates nothing (cs_hip_exclusive_scan_u32_static) -------------------------------------------
+// Its workspace is `scratch`, a file-scope device array (each device holds its own copy until the process ends), sized
+// for CS_HIP_SCAN_STATIC_MAX_N elements: the header and one status word per tile. Host code never passes the array's name
+// to a kernel (on the host it is only a stand-in): the first call on a device looks up the array's device address there
+// with hipGetSymbolAddress and keeps it; kernels and the one-time clearing get that address.
+constexpr uint32_t kStaticTiles = static_cast<uint32_t>((CS_HIP_SCAN_STATIC_MAX_N + kTile - 1) / kTile);
+constexpr size_t   kStaticBytes = sizeof(ScanHeader) + sizeof(uint64_t) * kStaticTiles;
+static_assert(kStaticBytes % sizeof(uint64_t) == 0, "the scratch array holds whole status words");
+static __device__ __align__(256) uint64_t scratch[kStaticBytes / sizeof(uint64_t)];
+
+constexpr int kMaxDevices = 64;
+std::mutex    g_static_mu;
+void*         g_static_ptr[kMaxDevices] = {};  // per device: the device address of `scratch` there (no host allocation)
+
+/// The address of the current device's `scratch`, looked up (and the array cleared, on stream `s`, by a fill kernel) on
+/// the first call on that device. Every scan leaves the array zeroed again, so later calls need no clearing.
+int static_workspace(cs_hip_stream s, void** out) {
+    int dev = 0;
+    if (int r = cs_hip::check(hipGetDevice(&dev), "hipGetDevice")) return r;
+    if (dev < 0 || dev >= kMaxDevices)
+        return cs_hip::fail(CS_HIP_ERR_INVALID, "cs_hip_exclusive_scan_u32_static: device %d is past the %d this library tracks", dev,
+                            kMaxDevices);
+    std::lock_guard<std::mutex> lock{g_static_mu};
+    if (g_static_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        if (int r = cs_hip::check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return r;
+        if (p == nullptr) return cs_hip::fail(CS_HIP_ERR_RUNTIME, "hipGetSymbolAddress(scratch) gave a null address");
+        if (int r = cs_hip_fill_u32_async(p, 0u, kStaticBytes / 4, s)) return r;
+        g_static_ptr[dev] = p;
+    }
+    *out = g_static_ptr[dev];
+    return CS_HIP_OK;
+}
+
 template <bool Column, class Src>
 int launch_scan(const Src* in, size_t kstride, uint32_t* extents, uint32_t* out, size_t n, uint32_t* result, void* workspace,
                 cs_hip_stream s) {
@@ -341,6 +375,18 @@ int cs_hip_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void*
     if (!out || (n != 0 && !in)) return cs_hip::fail(CS_HIP_ERR_INVALID, "cs_hip_exclusive_scan_u32: null pointer");
     if (n > (size_t{1} << 40)) return cs_hip::fail(CS_HIP_ERR_INVALID, "cs_hip_exclusive_scan_u32: n = %zu is too large", n);
     return launch_scan<false, uint32_t>(in, 1, nullptr, out, n, nullptr, workspace, s);
+}
+
+int cs_hip_exclusive_scan_u32_static(const uint32_t* in, uint32_t* out, size_t n, cs_hip_stream s) {
+    if (!out || (n != 0 && !in)) return cs_hip::fail(CS_HIP_ERR_INVALID, "cs_hip_exclusive_scan_u32_static: null pointer");
+    if (n > CS_HIP_SCAN_STATIC_MAX_N)
+        return cs_hip::fail(CS_HIP_ERR_INVALID,
+                            "cs_hip_exclusive_scan_u32_static: n = %zu; this scan's fixed workspace holds at most %llu elements "
+                            "(use cs_hip_exclusive_scan_u32 for more)",
+                            n, static_cast<unsigned long long>(CS_HIP_SCAN_STATIC_MAX_N));
+    void* ws = nullptr;
+    if (int r = static_workspace(s, &ws)) return r;
+    return launch_scan<false, uint32_t>(in, 1, nullptr, out, n, nullptr, ws, s);
 }
 
 int cs_hip_build_column(int src_dtype, const void* src, size_t count_src, size_t k, int offsets_mode, uint32_t* extents,

@@ -133,6 +133,16 @@ size_t cs_hip_scan_workspace_bytes(size_t n);
 int    cs_hip_scan_workspace_init(void* workspace, size_t bytes, cs_hip_stream s);
 int    cs_hip_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void* workspace, cs_hip_stream s);
 
+/* The same scan with no allocation of any kind. No counterpart in the reference: it is for callers that cannot allocate
+ * (code under graph capture, tight serving loops, harnesses that forbid hipMalloc). Its workspace is a fixed array the
+ * library holds on each device until the process ends (CS_HIP_SCAN_STATIC_BYTES), so n may be at most
+ * CS_HIP_SCAN_STATIC_MAX_N (larger n: CS_HIP_ERR_INVALID, nothing launched). The first call on a device clears the array
+ * with a kernel on stream s; every scan leaves it zeroed. All calls on one device share the array: calls that overlap on
+ * one device (two streams, two threads) race, so keep them on one stream. */
+#define CS_HIP_SCAN_STATIC_MAX_N (1ull << 26)
+#define CS_HIP_SCAN_STATIC_BYTES (16u + 8u * (unsigned)(CS_HIP_SCAN_STATIC_MAX_N / 2048u))
+int cs_hip_exclusive_scan_u32_static(const uint32_t* in, uint32_t* out, size_t n, cs_hip_stream s);
+
 /* ---- ragged column build --------------------------------------------------------------------------------------------
  * replaces: build_column (cospanmdao/include/naga/cospan/shape_hook.hpp:215-348) with its flag bits (:180-201).
  * src_dtype: 0 i32, 1 u32, 2 i64, 3 u64. counts mode (offsets_mode 0): src is [count_src, k] row-major, column 0

@@ -443,6 +443,10 @@ int    cs_hip_exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, vo
     });
     return CS_HIP_OK;
 }
+int cs_hip_exclusive_scan_u32_static(const uint32_t* in, uint32_t* out, size_t n, cs_hip_stream s) {
+    if (n > CS_HIP_SCAN_STATIC_MAX_N) return fail(CS_HIP_ERR_INVALID, "cs_hip_exclusive_scan_u32_static: n past the fixed workspace");
+    return cs_hip_exclusive_scan_u32(in, out, n, nullptr, s);
+}
 int cs_hip_build_column(int dt, const void* src, size_t count, size_t k, int offsets_mode, uint32_t* ext, uint32_t* offs, uint32_t* result,
                         void* ws, cs_hip_stream s) {
     if (!offs || !result || (count && (!src || !ext))) return fail(CS_HIP_ERR_INVALID, "cs_hip_build_column: null pointer");
