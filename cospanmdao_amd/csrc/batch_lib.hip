@@ -104,9 +104,9 @@ struct cs_batch_s {
 
 namespace {
 
-/// Whether cs_batch_set_mode may select the persistent work-queue form (RaggedGemver). Off until its restructured work
-/// loop has passed its GPU tests: an earlier form of the loop hung the GPU.
-constexpr bool kPersistentFormEnabled = false;
+/// Whether cs_batch_set_mode may select the persistent work-queue form (RaggedGemver). An earlier form of its work loop
+/// hung the GPU; the restructured loop (one leader-only block per iteration, between barriers) is on.
+constexpr bool kPersistentFormEnabled = true;
 
 cs::Model& model(cs_batch_s& g) {
     if (!g.model) {

@@ -11,15 +11,14 @@ from cospanmdao_amd import CsError, check, workloads
 
 pytestmark = pytest.mark.gpu
 
-# The persistent kernel hung on the device twice: first from a compiled loop nest in which the leader lane left the work
-# loop to fetch the next ticket while the other lanes re-read the previous one (fixed: a barrier at the loop top, checked
-# in the ISA), then because the library had not been rebuilt with that fix (the Makefile did not list the workload
-# headers; fixed). The fixed kernel has not run on a GPU yet, so its tests stay skipped until it has; the model's host
-# logic (both forms) is checked on the CPU by tests/test_host_logic.py. The phased form has no work queue and no waits
-# between work-groups.
-PERSISTENT = pytest.mark.skip(reason="persistent batched kernel: fix not yet verified on a GPU (see the comment above)")
+# The persistent kernel hung on the device twice in its first form: the compiler merged the leader-only completion of
+# one unit with the leader-only ticket fetch of the next across the loop's back edge (the leader parked in a nested loop
+# while the other lanes re-read the previous unit), and a rebuilt library with the first fix was not shipped (the
+# libraries now carry a build id that the tests compare with the tree). The work loop now holds ONE leader-only block per
+# iteration (completion and fetch together, between two barriers; checked in the ISA of this build).
+PERSISTENT = pytest.mark.gpu  # no extra condition: the persistent tests run
 # whether the library lets cs_batch_set_mode select the persistent form (batch_lib.hip kPersistentFormEnabled)
-PERSISTENT_ENABLED = False
+PERSISTENT_ENABLED = True
 PHASED, PERSISTENT_MODE = 0, 1
 
 ALPHA, BETA = 1.5, 1.2
