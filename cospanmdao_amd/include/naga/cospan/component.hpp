@@ -138,6 +138,23 @@ protected:
         add_port(&w, std::move(name), detail::IOKind::output, DT, R, shape,
                  [&w](void* p, const std::size_t* s, const std::size_t* st) { detail::WrapperAccess::point(w, p, s, st); });
     }
+    /// One spec per dimension, as the reference's declarations read (component.hpp:196-207): an extent or cs::unspecified
+    /// for each of the port's R >= 2 dimensions (rank 1 takes its one extent or cs::unspecified through ShapeSpec).
+    template <DType DT, int R, DimSpec... S>
+        requires(sizeof...(S) == R && R >= 2)
+    void declare_input(Input<DT, R>& w, std::string name, S... specs) {
+        declare_input(w, std::move(name), shape_of_specs(specs...));
+    }
+    template <DType DT, int R, DimSpec... S>
+        requires(sizeof...(S) == R && R >= 2)
+    void declare_output(Output<DT, R>& w, std::string name, S... specs) {
+        declare_output(w, std::move(name), shape_of_specs(specs...));
+    }
+    template <DType DT, int R, DimSpec... S>
+        requires(sizeof...(S) == R && R >= 2)
+    void declare_scratch(Output<DT, R>& w, std::string name, S... specs) {
+        declare_scratch(w, std::move(name), shape_of_specs(specs...));
+    }
     /// An output the component keeps as device state between forwards (work counters, look-back status words): zeroed
     /// whenever it is (re)allocated, and left zeroed by the kernels that use it, so every forward (and every replay of a
     /// captured graph) starts from zero without a memset.

@@ -18,6 +18,7 @@
 #include <initializer_list>
 #include <memory>
 #include <optional>
+#include <concepts>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -66,6 +67,29 @@ ShapeSpec unspecified_shape() {
     ShapeSpec s;
     s.rank = R;
     return s;
+}
+
+/// What one dimension of a declaration may say (reference metadata.hpp:54): an extent, or cs::unspecified.
+template <class T>
+concept DimSpec = std::integral<T> || std::same_as<T, UnspecifiedTag>;
+
+/// The shape a declaration with one spec per dimension gives (declare_output(y, "y", 4, ny, cs::unspecified)).
+template <DimSpec... S>
+ShapeSpec shape_of_specs(S... specs) {
+    ShapeSpec out;
+    out.rank = static_cast<int>(sizeof...(S));
+    int  d   = 0;
+    auto put = [&](auto v) {
+        if constexpr (std::integral<decltype(v)>) {
+            if constexpr (std::is_signed_v<decltype(v)>)
+                if (v < 0) throw std::invalid_argument("naga::cospan: a declared extent is negative");
+            out.extents[d]   = static_cast<std::size_t>(v);
+            out.specified[d] = true;
+        }
+        ++d;
+    };
+    (put(specs), ...);
+    return out;
 }
 
 /// A fully specified shape (cs::shape(1), cs::shape(n, m)).

@@ -46,3 +46,20 @@ def test_ragged_gemver_program():
     ragged_gemver.cpp): bare and fused passes, sizes and instance counts changing between forwards, every instance against
     FP64 (rel-L2 <= 1e-5)."""
     run_program("ragged_gemver")
+
+
+def test_range_split_program():
+    """The range_split strategies and launch shapes (tests/framework/range_split.cpp, after the reference's
+    tests/test_range_split.cpp:569-667): more than 65535 groups per stage under each strategy, a stage taller than a
+    hardware grid's 65535 in y, bare group-range and index-range components of 67600 groups, rank-3 and rank-1 stages, a
+    bare component's native launch, the physical group coordinates of each strategy, padded refusing rank 3."""
+    run_program("range_split")
+
+
+def test_getting_started_demo():
+    """examples/getting_started.cpp (the reference's getting-started/main.cpp with the three changes of INTEGRATION.md),
+    built with INTEGRATION.md's build line, prints y = 2 x + 1 for x = 1, 2, 3, 4 on the GPU."""
+    exe = os.path.join(HERE, "framework", "_build", "getting_started")
+    assert os.path.exists(exe), "tests/framework/_build/getting_started is not built (make -C tests/framework)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and r.stdout.split() == ["3", "5", "7", "9"], r.stdout + r.stderr

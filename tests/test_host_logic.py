@@ -68,3 +68,11 @@ def test_policies_host_logic():
     build()
     out = run("policies_host")
     assert "depends_on: B reads what A wrote" in out and "transient" in out
+
+
+def test_getting_started_demo_on_the_cpu_boundary():
+    """examples/getting_started.cpp, the reference's one-component demo, included as it is and run against the CPU
+    boundary with its kernel emulated by the user's own compute_forward: it prints y = 2 x + 1 for x = 1, 2, 3, 4."""
+    build()
+    r = subprocess.run([os.path.join(HOST, "_build", "getting_started_host")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.split() == ["3", "5", "7", "9"], r.stdout + r.stderr
