@@ -252,21 +252,24 @@ def bench_nsc(extra, iters, sizes=(1024, 2048, 4096)):
 
 
 def bench_precision(extra, steps):
-    """GEMVER N = 32768 with A stored f32, bf16 and e4m3 + row scale, each at its fastest pass-2 tile height (4 or 8 rows,
-    both recorded): the speedups compare the storage types at their own best."""
+    """GEMVER N = 32768 with A stored f32, bf16 and e4m3 + row scale, each at its fastest strip height R (128, 256, 512:
+    pass 1 runs best with about two work-groups per CU, and R sets their number) and pass-2 tile height (4 or 8 rows); every
+    combination is recorded, and the speedups compare the storage types at their own best."""
     n = 32768
     rec = {}
     for store, name, elem in ((0, "f32", 4), (1, "bf16", 2), (2, "e4m3", 1)):
         nbytes = gemver_bytes_fused(n, elem) + (n * 4 if store == 2 else 0)
         tried = {}
-        for rt2 in (4, 8):
-            g = Gemver(TWOPASS, n, 128, store=store, rt2=rt2)
-            g.set_replay(REPLAY_DIRECT)
-            ms = timed_steps(g, steps, 3)
-            tried[rt2] = {"ms": round(ms, 4), "GBps": round(gbs(nbytes, ms), 1), "launch_ms": [round(t, 4) for t in g.launches(3)]}
-            g.close()
+        for R in (128, 256, 512):
+            for rt2 in (4, 8):
+                g = Gemver(TWOPASS, n, R, store=store, rt2=rt2)
+                g.set_replay(REPLAY_DIRECT)
+                ms = timed_steps(g, steps, 3)
+                tried[f"R{R}_rt{rt2}"] = {"ms": round(ms, 4), "GBps": round(gbs(nbytes, ms), 1), "R": R, "pass2_rows": rt2,
+                                          "launch_ms": [round(t, 4) for t in g.launches(3)]}
+                g.close()
         best = min(tried, key=lambda k: tried[k]["ms"])
-        rec[name] = dict(tried[best], pass2_rows=best, algorithmic_bytes=nbytes, tried={str(k): v for k, v in tried.items()})
+        rec[name] = dict(tried[best], algorithmic_bytes=nbytes, tried=tried)
     rec["bf16_speedup_vs_f32"] = round(rec["f32"]["ms"] / rec["bf16"]["ms"], 3)
     rec["e4m3_speedup_vs_f32"] = round(rec["f32"]["ms"] / rec["e4m3"]["ms"], 3)
     extra["gemver32768_storage"] = rec
