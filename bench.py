@@ -456,6 +456,8 @@ def main():
     ap.add_argument("--nsc-sizes", default="1024,2048,4096", help="grid sizes n (nx = ny) of --nsc-only, comma separated")
     ap.add_argument("--chain-only", action="store_true",
                     help="only the GEMVER -> BiCGK -> GEMVER chain at n = 16384 in six styles; prints its record, not the bench line")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the GEMVER N=16384 as the reference's four unfused components; prints its record, not the bench line")
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched ragged GEMVER (512 instances, N in [64, 4096]); prints its record, not the bench line")
     a = ap.parse_args()
@@ -487,6 +489,14 @@ def main():
         extra = {}
         bench_chain(extra, max(10, a.steps // 10))
         print(json.dumps(extra["gemver_chain"]), flush=True)
+        return
+    if a.plain_only:
+        p = Gemver(UNFUSED, a.n, 64)
+        pms = timed_steps(p, a.steps, 3)
+        ln = p.launches(3)
+        p.close()
+        print(json.dumps({"plain_ms": round(pms, 4), "GBps_own_bytes": round(gbs(gemver_bytes_plain(a.n, 64), pms), 1),
+                          "launch_ms": [round(t, 4) for t in ln]}), flush=True)
         return
     if a.batch_only:
         extra = {}
