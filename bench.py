@@ -253,15 +253,15 @@ def bench_nsc(extra, iters, sizes=(1024, 2048, 4096)):
 
 def bench_precision(extra, steps):
     """GEMVER N = 32768 with A stored f32, bf16 and e4m3 + row scale, each at its fastest strip height R (128, 256, 512:
-    pass 1 runs best with about two work-groups per CU, and R sets their number) and pass-2 tile height (4 or 8 rows); every
-    combination is recorded, and the speedups compare the storage types at their own best."""
+    pass 1 runs best with about two work-groups per CU, and R sets their number) and pass-2 tile height (4, 8, 16 or 32
+    rows); every combination is recorded, and the speedups compare the storage types at their own best."""
     n = 32768
     rec = {}
     for store, name, elem in ((0, "f32", 4), (1, "bf16", 2), (2, "e4m3", 1)):
         nbytes = gemver_bytes_fused(n, elem) + (n * 4 if store == 2 else 0)
         tried = {}
         for R in (128, 256, 512):
-            for rt2 in (4, 8):
+            for rt2 in (4, 8, 16, 32):
                 g = Gemver(TWOPASS, n, R, store=store, rt2=rt2)
                 g.set_replay(REPLAY_DIRECT)
                 ms = timed_steps(g, steps, 3)
@@ -276,7 +276,7 @@ def bench_precision(extra, steps):
     log(f"gemver N=32768 storage: {rec}")
 
 
-def bench_cache_cold(extra, steps, n=16384, R=128, rt2=4):
+def bench_cache_cold(extra, steps, n=16384, R=128, rt2=16):
     """The headline workload with A not left in any cache by the step before: two and four models (separate 1 GiB copies
     of A, different seeds) run in turn, so every step reads an A that the last step did not touch (the last-level cache
     holds 256 MB). Per-step rate of the rotation, and the dominant pass's per-launch time inside the rotation (device
@@ -441,7 +441,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--R", type=int, default=128)
-    ap.add_argument("--rt2", type=int, default=4)
+    ap.add_argument("--rt2", type=int, default=16, help="rows per pass-2 work-group (4, 8, 16, 32)")
     ap.add_argument("--style", default="twopass", choices=["twopass", "twopass_1"])
     ap.add_argument("--quick", action="store_true", help="the headline workload only (no secondary workloads, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")

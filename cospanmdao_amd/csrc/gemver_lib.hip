@@ -46,6 +46,7 @@ std::unique_ptr<cs::Model> build_rt(const cs_gemver_s& g) {
     switch (g.rt2) {
         case 4: return build_model<S, 4>(g);
         case 16: return build_model<S, 16>(g);
+        case 32: return build_model<S, 32>(g);
         default: return build_model<S, 8>(g);
     }
 }
@@ -95,7 +96,8 @@ int cs_gemver_create(cs_gemver* out, int style, int store, size_t n, size_t R, f
 int cs_gemver_set_tile(cs_gemver g, int pass2_rows) {
     return wl::guard([&] {
         if (!g) throw std::invalid_argument("cs_gemver_set_tile: null handle");
-        if (pass2_rows != 4 && pass2_rows != 8 && pass2_rows != 16) throw std::invalid_argument("gemver: pass-2 rows are 4, 8 or 16");
+        if (pass2_rows != 4 && pass2_rows != 8 && pass2_rows != 16 && pass2_rows != 32)
+            throw std::invalid_argument("gemver: pass-2 rows are 4, 8, 16 or 32");
         if (g->model) throw std::logic_error("gemver: the tile is fixed once the model has run");
         g->rt2 = pass2_rows;
     });

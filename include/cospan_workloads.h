@@ -43,7 +43,8 @@ const char* cs_workloads_build_id(void);
 typedef struct cs_gemver_s* cs_gemver;
 
 int cs_gemver_create(cs_gemver* g, int style, int store, size_t n, size_t R, float alpha, float beta, uint32_t base_seed);
-/* the work-group shapes of the TWOPASS styles: rows per pass-2 work-group (4, 8 or 16) */
+/* the work-group shapes of the TWOPASS styles: rows per pass-2 work-group (4 or 8: the rows spread over the work-group's
+ * four waves; 16 or 32: each wave owns a quarter of the rows and reduces them alone) */
 int cs_gemver_set_tile(cs_gemver g, int pass2_rows);
 int cs_gemver_run(cs_gemver g);
 int cs_gemver_wait(cs_gemver g);

@@ -77,7 +77,7 @@ def test_reference_styles_bitwise(style, n, R):
         g.close()
 
 
-@pytest.mark.parametrize("rt2", [4, 8, 16])
+@pytest.mark.parametrize("rt2", [4, 8, 16, 32])
 @pytest.mark.parametrize("n,R", [(256, 64), (1024, 64), (1040, 64), (2048, 256), (4096, 128)])
 @pytest.mark.parametrize("style", ["twopass", "twopass_1"])
 def test_twopass_f32(style, n, R, rt2):
@@ -117,13 +117,15 @@ def test_twopass_full_size():
     assert ex < TOL_L2 and ew < TOL_L2
 
 
+@pytest.mark.parametrize("rt2", [8, 16, 32])
 @pytest.mark.parametrize("store", [1, 2])
 @pytest.mark.parametrize("n", [1024, 2064])
-def test_twopass_reduced_precision(store, n):
+def test_twopass_reduced_precision(store, n, rt2):
     """A stored bf16 (store 1) or e4m3 with per-row f32 scale (store 2): the oracle is FP64 on the dequantised stored A;
-    the quantisation itself is checked bit for bit against the CPU quantiser."""
-    print("gemver reduced", store, n, flush=True)
-    g = Gemver(TWOPASS, n, 64, store=store)
+    the quantisation itself is checked bit for bit against the CPU quantiser. Pass-2 tiles of 8 rows (spread over the
+    work-group) and 16, 32 (a quarter per wave; 2064 rows leave a partial last tile at 32)."""
+    print("gemver reduced", store, n, rt2, flush=True)
+    g = Gemver(TWOPASS, n, 64, store=store, rt2=rt2)
     try:
         g.run()
         inp = O.gemver_inputs(n)
@@ -176,3 +178,9 @@ def test_bad_arguments_are_errors():
     assert workloads.cs_gemver_create(C.byref(h), UNFUSED, 0, 1000, 64, 1.0, 1.0, 0) == 1  # n % 256 != 0
     assert workloads.cs_gemver_create(C.byref(h), UNFUSED, 1, 1024, 64, 1.0, 1.0, 0) == 1  # bf16 needs two-pass
     assert workloads.cs_gemver_create(C.byref(h), 9, 0, 1024, 64, 1.0, 1.0, 0) == 1
+    check(workloads.cs_gemver_create(C.byref(h), TWOPASS, 0, 1024, 64, 1.0, 1.0, 0))
+    try:
+        for rows in (0, 3, 12, 64):  # pass-2 tiles are 4, 8, 16 or 32 rows
+            assert workloads.cs_gemver_set_tile(h, rows) == 1, rows
+    finally:
+        check(workloads.cs_gemver_destroy(h))

@@ -1,11 +1,11 @@
 """Parity at the full sizes of the measured configurations, with the per-output errors against the FP64 oracle recorded
 (tests/helpers.py record_errors; the committed summary is profiles/parity_errors_full_size.md):
 
-  * GEMVER N = 16384 f32, fused two-pass and the reference's four unfused components (B, P, x, w bit for bit against
-    the f32 restatement of the components, x and w within 1e-5 rel-L2 of FP64);
-  * GEMVER N = 32768 with A stored bf16 and e4m3 + per-row f32 scale: the stored A bit for bit against the CPU
-    quantisers, x and w within 1e-5 rel-L2 of FP64 on the dequantised stored A (the bar), at both pass-2 tile heights;
-    the quantisation error (FP64 on the original f32 A) recorded beside it;
+  * GEMVER N = 16384 f32, fused two-pass (pass-2 tiles of 16 and 4 rows) and the reference's four unfused components
+    (B, P, x, w bit for bit against the f32 restatement of the components, x and w within 1e-5 rel-L2 of FP64);
+  * GEMVER N = 32768 with A stored f32, bf16 and e4m3 + per-row f32 scale: the stored A bit for bit against the CPU
+    quantisers, x and w within 1e-5 rel-L2 of FP64 on the dequantised stored A (the bar), at every strip height R and
+    pass-2 tile the benchmark picks from; the quantisation error (FP64 on the original f32 A) recorded beside it;
   * the NS residual at 4096^2 (f32, fused and unfused), 8192^2 (f32) and 8192^2 with bf16 fields: Ru, Rv, Rc within 1e-5
     rel-L2 of FP64 (on the stored fields), the FP64 norms within 1e-5 relative, the quantisation error of bf16 fields
     recorded.
@@ -60,13 +60,21 @@ def test_gemver_f32_n16384_fused_and_unfused():
     xr, wr = O.gemver_f64(inp, ALPHA, BETA)
     rec = {}
     print("gemver twopass n", n, flush=True)
-    g = Gemver(TWOPASS, n, 128, rt2=4)
+    g = Gemver(TWOPASS, n, 128, rt2=16)
     try:
         g.run(times=3)  # eager, captured, replayed
         gx, gw = g.read("x", np.float32, n), g.read("w", np.float32, n)
     finally:
         g.close()
-    rec["twopass"] = {"x": errors(gx, xr), "w": errors(gw, wr), "R": 128, "pass2_rows": 4}
+    rec["twopass"] = {"x": errors(gx, xr), "w": errors(gw, wr), "R": 128, "pass2_rows": 16}
+    g = Gemver(TWOPASS, n, 128, rt2=4)
+    try:
+        g.run(times=3)
+        x4, w4 = g.read("x", np.float32, n), g.read("w", np.float32, n)
+    finally:
+        g.close()
+    assert np.array_equal(x4, gx), "x does not depend on the pass-2 tile"
+    rec["twopass_pass2_rows_4"] = {"w": errors(w4, wr)}
     print("gemver unfused n", n, flush=True)
     g = Gemver(UNFUSED, n, 64)
     try:
@@ -112,7 +120,7 @@ def test_gemver_n32768_reduced_precision_storage(store, name):
         L.or_quantize_e4m3_rows(O.ptr(inp["A"]), O.ptr(q), O.ptr(scale), n, n, n, n)
     rec = {"n": n, "storage": name}
     outs = {}
-    for R, rt2 in ((128, 4), (128, 8), (256, 8), (512, 8)):  # the strip and pass-2 tile heights the benchmark chooses among
+    for R, rt2 in [(R, t) for R in (128, 256, 512) for t in (4, 8, 16, 32)]:  # every pair the benchmark chooses among
         print("gemver", name, "n", n, "R", R, "rt2", rt2, flush=True)
         g = Gemver(TWOPASS, n, R, store=store, rt2=rt2)
         try:
@@ -124,7 +132,8 @@ def test_gemver_n32768_reduced_precision_storage(store, name):
             outs[(R, rt2)] = (g.read("x", np.float32, n), g.read("w", np.float32, n))
         finally:
             g.close()
-    assert np.array_equal(outs[(128, 4)][0], outs[(128, 8)][0]), "x does not depend on the pass-2 tile"
+    for R, rt2 in outs:
+        assert np.array_equal(outs[(R, 4)][0], outs[(R, rt2)][0]), ("x does not depend on the pass-2 tile", R, rt2)
     xs, ws = O.gemver_f64(inp, ALPHA, BETA, store=store, A=q, row_scale=scale)  # FP64 on the dequantised stored A
     del q
     xo, wo = O.gemver_f64(inp, ALPHA, BETA) if store else (xs, ws)  # FP64 on the original f32 A
