@@ -337,11 +337,11 @@ def batch_form(mode, reps):
     st = (C.c_uint64 * 4)()
     check(W.cs_batch_stats(h, st))
     phase_ms = []
-    if mode == 0:
-        for leaf in (2, 3, 4, 5):  # unit map, pass 1, column reductions, pass 2, each back to back
-            t = C.c_float()
-            check(W.cs_batch_time_stage(h, leaf, 10, C.byref(t)))
-            phase_ms.append(round(t.value, 4))
+    # unit map, then pass 1, column reductions, pass 2 (phased) or the one work-queue launch (persistent), back to back
+    for leaf in ((2, 3, 4, 5) if mode == 0 else (2, 3)):
+        t = C.c_float()
+        check(W.cs_batch_time_stage(h, leaf, 10, C.byref(t)))
+        phase_ms.append(round(t.value, 4))
     check(W.cs_batch_destroy(h))
     n = sizes.astype(np.float64)
     nbytes = float(np.sum(2 * n * n * 4 + 13 * n * 4))
@@ -356,15 +356,17 @@ def batch_form(mode, reps):
 
 
 def bench_batch(extra, fused_gbps, reps):
-    """The batched ragged GEMVER in both forms: the persistent one (plan, device scan, then ONE work-queue launch that runs
-    every instance's pass 1, column reductions and pass 2) and the phased one (plan, scan, unit map, then one launch per
-    phase); each replayed as one captured graph. The line reports the faster; both are kept."""
+    """The batched ragged GEMVER in both forms: the persistent one (plan, device scan, unit map, then ONE work-queue launch
+    that runs every instance's pass 1, column reductions and pass 2) and the phased one (plan, scan, unit map, then one
+    launch per phase); each replayed as one captured graph. The line reports the faster; both are kept."""
     forms = {"persistent": batch_form(1, reps), "phased": batch_form(0, reps)}
-    forms["persistent"]["form"] = "persistent: plan, device scan, one work-queue launch (three launches, one replayed graph)"
+    forms["persistent"]["form"] = ("persistent: plan, device scan, unit map, one work-queue launch for every unit of the three "
+                                   "phases (four launches, one replayed graph)")
     forms["phased"]["form"] = "phased: plan, device scan, unit map, pass 1, column reductions, pass 2 (six launches, one replayed graph)"
+    for f in forms.values():
+        f["per_byte_rate_vs_fused_16384"] = round(f["GBps"] / fused_gbps, 3) if fused_gbps else None
     best = min(forms, key=lambda k: forms[k]["ms_mean"])
     rec = dict(forms[best])
-    rec["per_byte_rate_vs_fused_16384"] = round(rec["GBps"] / fused_gbps, 3) if fused_gbps else None
     rec["forms"] = forms
     extra["batched_ragged"] = rec
     log(f"batched: best {best}; " + "; ".join(f"{k} {v['ms_mean']:.3f} ms (p99 {v['ms_p99']:.3f})" for k, v in forms.items()))

@@ -2,7 +2,7 @@
 //
 // A handle owns a cs::Model whose root is workloads/batch_systems.hpp's BatchedGemver, in its phased form (plan, scan,
 // unit map, pass 1, column reductions, pass 2: six launches) unless cs_batch_set_mode asks for the persistent one (plan,
-// scan, one work-queue kernel). Runs after the first two are replays of one captured graph. The autotuner times every candidate
+// scan, unit map, then every unit of the three phases in one work-queue kernel). Runs after the first two are replays of one captured graph. The autotuner times every candidate
 // (strip rows R, pass-2 rows RT and, persistent form only, work-groups G) on the model itself and keeps the fastest;
 // results are cached per device, problem and form in a small text file so later processes skip the search.
 
@@ -104,8 +104,7 @@ struct cs_batch_s {
 
 namespace {
 
-/// Whether cs_batch_set_mode may select the persistent work-queue form (RaggedGemver). An earlier form of its work loop
-/// hung the GPU; the restructured loop (one leader-only block per iteration, between barriers) is on.
+/// Whether cs_batch_set_mode may select the persistent work-queue form (RaggedGemver).
 constexpr bool kPersistentFormEnabled = true;
 
 cs::Model& model(cs_batch_s& g) {
@@ -121,8 +120,7 @@ cs::Model& model(cs_batch_s& g) {
 
 void set_params(cs_batch_s& g, std::uint32_t R, std::uint32_t RT, std::uint32_t G) {
     if (R < batch::BatchedGemver::kMinR) throw std::invalid_argument("batch: R must be at least 32 (the partials are sized for it)");
-    if (g.persistent ? (RT != 4 && RT != 8) : (RT != 4 && RT != 8 && RT != 16 && RT != 32))
-        throw std::invalid_argument(g.persistent ? "batch: RT is 4 or 8 (persistent form)" : "batch: RT is 4, 8, 16 or 32");
+    if (RT != 4 && RT != 8 && RT != 16 && RT != 32) throw std::invalid_argument("batch: RT is 4, 8, 16 or 32");
     if (G < 1 || G > 65536) throw std::invalid_argument("batch: the persistent grid is 1 .. 65536 work-groups");
     g.params = {R, RT, G};
     g.knobs->groups = static_cast<int>(G);
@@ -209,15 +207,11 @@ int cs_batch_set_mode(cs_batch g, int persistent) {
         if (!g) throw std::invalid_argument("cs_batch_set_mode: null handle");
         if (persistent != 0 && persistent != 1) throw std::invalid_argument("cs_batch_set_mode: mode is 0 (phased) or 1 (persistent)");
         if (persistent == 1 && !kPersistentFormEnabled)
-            throw std::invalid_argument(
-                "cs_batch_set_mode: the persistent form is switched off: its work loop hung on the GPU (a leader-only "
-                "completion and the next leader-only fetch merged across the loop's back edge); the restructured loop has "
-                "not yet run on a GPU");
+            throw std::invalid_argument("cs_batch_set_mode: the persistent form is switched off in this build");
         if ((persistent != 0) != g->persistent) {
             if (g->model) g->model->wait();
             g->model.reset();  // rebuilt in the other form at the next use
             g->persistent = persistent != 0;
-            if (g->persistent && g->params[1] > 8) g->params[1] = 8;  // the persistent form takes RT 4 or 8
             batch::count_units(g->layout->sizes.data(), g->layout->sizes.size(), g->params[0], g->params[1], g->knobs->units);
         }
     });
@@ -302,8 +296,8 @@ int cs_batch_autotune(cs_batch g, int reps, const char* cache_path, uint32_t* be
             float         best = 1e30f;
             std::uint32_t bp[3] = {64, 8, static_cast<std::uint32_t>(4 * cus)};
             const std::vector<std::uint32_t> gms = g->persistent ? std::vector<std::uint32_t>{2u, 4u, 8u} : std::vector<std::uint32_t>{4u};
-            const std::vector<std::uint32_t> Rs  = g->persistent ? std::vector<std::uint32_t>{32u, 64u, 128u} : std::vector<std::uint32_t>{64u, 128u, 256u};
-            const std::vector<std::uint32_t> RTs = g->persistent ? std::vector<std::uint32_t>{4u, 8u} : std::vector<std::uint32_t>{8u, 16u, 32u};
+            const std::vector<std::uint32_t> Rs{64u, 128u, 256u};
+            const std::vector<std::uint32_t> RTs{8u, 16u, 32u};
             for (std::uint32_t R : Rs)
                 for (std::uint32_t RT : RTs)
                     for (std::uint32_t gm : gms) {

@@ -8,33 +8,39 @@
 // 16-byte aligned), vectors u1 v1 u2 v2 y z of N_k, outputs x_k = beta Â_k^T y_k + z_k and w_k = alpha Â_k x_k. Every
 // instance's arrays sit back to back in one allocation per array; where each starts comes from a DEVICE exclusive scan:
 //
-//   BatchPlan      (index range over K) per instance six counts, stored column-major [6][K]:
+//   BatchPlan      (index range over K) per instance nine counts, stored column-major [9][K]:
 //                    0 V  = N_k                                   vector elements          (x, w, u1 .. z offsets)
 //                    1 A4 = N_k * ld_k / 4                        16-byte units of A_k     (A offsets)
 //                    2 P  = strips_k * N_k                        strip partials           (P offsets)
 //                    3 U1 = strips_k * ceil(N_k / 4096)           pass-1 tiles (R rows x 4096 columns)
 //                    4 UR = ceil(N_k / 1024)                      column-reduction blocks
 //                    5 U2 = ceil(N_k / RT)                        pass-2 row groups
-//                  (strips_k = ceil(N_k / R))
-//   ColumnScan     (custom op) ONE u32 exclusive scan of the 6K counts (cs_hip_build_column: decoupled look-back, wrap
+//                    6, 7, 8  U1, UR, U2 again, at the instance's place in the work order (rank_k) instead of at k
+//                  (strips_k = ceil(N_k / R)). The work order runs the largest instances first (the host ranks them by
+//                  size), so that every phase ends on small units: the data stay in instance order, only the units'
+//                  places in each phase's queue follow the ranks.
+//   ColumnScan     (custom op) ONE u32 exclusive scan of the 9K counts (cs_hip_build_column: decoupled look-back, wrap
 //                  flag): column c's offsets are scan[cK + k] - scan[cK]; column 0 is the int32 scan of the sizes.
-// then one of two forms:
-//   phased (default)
+// then one of two forms, both reading their units from
 //   UnitMap        (K work-groups) writes, for every unit of every phase, what the unit needs to start (instance size,
 //                  unit index within the instance, the instance's vector / A / P offsets) at the unit's scan position:
-//                  a phase's work-group then finds its unit with one load instead of a search over the scan.
+//                  a unit is found with one load instead of a search over the scan.
+//   phased (default)
 //   RaggedPass1 / RaggedReduce / RaggedPass2  one plain launch per phase, work-group u runs that phase's unit u: no
 //                  work queue and no waits between work-groups; the launch boundaries order the phases.
 //   persistent
-//   RaggedGemver   a grid of G work-groups fetches work units by ticket (start order) until none is left: first every
-//                  pass-1 tile of every instance, then every column reduction, then every pass-2 row group. A reduction of
-//                  instance k waits until all of k's pass-1 tiles are done, a pass-2 unit until all of k's reductions
-//                  are: every wait is on units with earlier tickets, held by work-groups that are running them and never
-//                  wait on later ones, so the queue drains without co-residency. The last work-group to leave zeroes the
-//                  counters (graph replays need no memset).
+//   RaggedGemver   ONE launch: a grid of G work-groups takes units by ticket until none is left, first every pass-1 tile of
+//                  every instance, then every column reduction, then every pass-2 row group (the queue protocol below).
+//                  A work-group publishes the units it ran when it first takes a unit of a later phase (one release, one
+//                  add per phase counter), and waits once per phase for the phase before it to be complete (one acquire):
+//                  at most two releases and two acquires per work-group, none per unit. Waits are only ever on units with
+//                  earlier tickets, held by work-groups that are running and publish them before they wait themselves, so
+//                  the queue drains without co-residency. The last work-group to leave zeroes the counters (graph
+//                  replays need no memset).
 // The arithmetic of a unit is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
-// the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4 or 8; the phased form also 16 or
-// 32) and G are run-time parameters (the autotuner's knobs; G only for the persistent form).
+// the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4, 8: the work-group shares a row
+// tile; 16, 32: each wave owns RT / 4 rows) and G are run-time parameters (the autotuner's knobs; G only for the
+// persistent form).
 
 #include <cstdint>
 #include <memory>
@@ -54,6 +60,7 @@ inline constexpr int kQ1    = 4;          // chunks of 4 columns per work-item i
 inline constexpr int kTile1 = kCols * kQ1;  // columns of one pass-1 tile: kQ1 interleaved chunks of kCols
 inline constexpr int kMaxRT = 8;
 inline constexpr int kMapWords = 8;       // u32 words per unit-map entry
+inline constexpr int kColumns  = 9;       // count columns per instance (the plan)
 
 __host__ __device__ inline std::uint32_t ceil_div(std::uint32_t a, std::uint32_t b) { return (a + b - 1) / b; }
 __host__ __device__ inline std::uint32_t ld_of(std::uint32_t n) { return (n + 3u) & ~3u; }
@@ -82,7 +89,7 @@ struct Unit {
     std::uint32_t n, t, vo, a4, poff;
 };
 
-/// The device scan's six offset columns (column c of instance k = scan[cK + k] - scan[cK]).
+/// The device scan's offset columns (column c of instance k = scan[cK + k] - scan[cK]).
 struct Cols {
     const std::uint32_t* s;
     std::uint32_t        K;
@@ -195,16 +202,42 @@ __device__ inline void unit_pass1(const Arrays& a, const Unit& wu, std::uint32_t
             if (jb + q * kCols + e < n) P[jb + q * kCols + e] = acc[q][e];
 }
 
-/// column-reduction block b of an instance: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b
-__device__ inline void unit_reduce(const Arrays& a, const Unit& w, std::uint32_t tid) {
-    const std::uint32_t n = w.n, b = w.t, strips = ceil_div(n, a.R);
+/// column-reduction block b of an instance: x[j] = fma(beta, sum_c P[c][j], z[j]) for the columns of block b. A
+/// work-item owns the block's columns tid + 256 q (q < 4) and loads 8 strips of all four at once (32 loads in flight);
+/// each column's sum still runs over the strips in ascending order.
+__device__ inline void unit_reduce(const Arrays& a, const Unit& wu, std::uint32_t tid) {
+    constexpr int       kQ = kCols / kWg, kS = 8;
+    const Unit          w{detail::uniform(wu.n), detail::uniform(wu.t), detail::uniform(wu.vo), detail::uniform(wu.a4), detail::uniform(wu.poff)};
+    const std::uint32_t n = w.n, strips = ceil_div(n, a.R);
     const float*        P = a.P_in + w.poff;
-    for (std::uint32_t j = b * kCols + tid; j < n && j < (b + 1) * kCols; j += kWg) {
-        float acc = 0.0f;
-#pragma unroll 8
-        for (std::uint32_t c = 0; c < strips; ++c) acc += P[static_cast<std::size_t>(c) * n + j];
-        a.x[w.vo + j] = fmaf(a.beta, acc, a.z[w.vo + j]);
+    std::uint32_t       j[kQ];
+    bool                ok[kQ];
+    float               acc[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+        j[q]   = w.t * kCols + tid + q * kWg;
+        ok[q]  = j[q] < n;
+        j[q]   = ok[q] ? j[q] : 0u;
+        acc[q] = 0.0f;
     }
+    std::uint32_t c = 0;
+    for (; c + kS <= strips; c += kS) {
+        float v[kS][kQ];
+#pragma unroll
+        for (int s = 0; s < kS; ++s)
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) v[s][q] = P[static_cast<std::size_t>(c + s) * n + j[q]];
+#pragma unroll
+        for (int s = 0; s < kS; ++s)
+#pragma unroll
+            for (int q = 0; q < kQ; ++q) acc[q] += v[s][q];
+    }
+    for (; c < strips; ++c)
+#pragma unroll
+        for (int q = 0; q < kQ; ++q) acc[q] += P[static_cast<std::size_t>(c) * n + j[q]];
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+        if (ok[q]) a.x[w.vo + j[q]] = fmaf(a.beta, acc[q], a.z[w.vo + j[q]]);
 }
 
 /// pass-2 row group g = t of an instance: rows [g RT, g RT + RT). Ends with a barrier (its LDS is reused by the next
@@ -321,14 +354,75 @@ __device__ inline bool map_unit(const std::uint32_t* map, std::size_t map_elems,
     return u < total && static_cast<std::size_t>(total) * kMapWords <= map_elems;
 }
 
+/// The persistent form's work queue, as host-and-device functions (the kernel runs them, and so does its CPU emulation,
+/// which interleaves simulated work-groups adversarially). Tickets run through three phases: [0, T1) pass-1 tiles,
+/// [T1, T1 + TR) column reductions, [T1 + TR, T1 + TR + T2) pass-2 row groups; any later ticket means "leave". A
+/// reduction of instance k needs all of k's pass-1 tiles and a pass-2 group all of k's reductions; the queue orders
+/// whole phases instead (a work-group's first unit of phase p waits until phase p - 1 is complete), which needs no
+/// per-unit release: state word done(q) counts the phase-q units published so far. The four counters sit 256 bytes apart,
+/// so the work-groups polling a done counter at a phase's end do not contend with the ticket fetches of those still working.
+namespace queue {
+inline constexpr int kStride = 64;  ///< u32 words between two counters
+inline constexpr int kTicket = 0, kFinished = kStride, kWords = 4 * kStride;
+CS_HD constexpr int done(int q) { return (2 + q) * kStride; }
+
+struct Totals {
+    std::uint32_t T1 = 0, TR = 0, T2 = 0;
+    CS_HD std::uint32_t all() const { return T1 + TR + T2; }
+    /// units of phase q (0: pass 1, 1: reductions)
+    CS_HD std::uint32_t of(int q) const { return q == 0 ? T1 : TR; }
+    /// the first ticket of phase p
+    CS_HD std::uint32_t base(int p) const { return p == 0 ? 0u : p == 1 ? T1 : T1 + TR; }
+};
+
+/// 0, 1, 2 for a ticket of that phase, 3 past the last unit.
+CS_HD inline int phase_of(std::uint32_t t, const Totals& n) {
+    if (t < n.T1) return 0;
+    t -= n.T1;
+    if (t < n.TR) return 1;
+    t -= n.TR;
+    return t < n.T2 ? 2 : 3;
+}
+
+/// What a work-group's leader keeps between units: the units of phases 0 and 1 it ran and has not published yet, and the
+/// phases it has seen complete (bit q).
+struct Leader {
+    std::uint32_t held[2] = {0, 0};
+    std::uint32_t seen    = 0;
+};
+
+/// Before a unit of phase p (p = 3: before leaving), the counts the leader publishes now: every held unit of a phase
+/// before p. True when there is any (the leader then releases once and adds add[q] to state word done(q)).
+CS_HD inline bool to_publish(Leader& L, int p, std::uint32_t (&add)[2]) {
+    add[0] = p >= 1 ? L.held[0] : 0u;
+    add[1] = p >= 2 ? L.held[1] : 0u;
+    L.held[0] -= add[0];
+    L.held[1] -= add[1];
+    return (add[0] | add[1]) != 0u;
+}
+
+/// The phase that must be complete before a unit of phase p runs, or -1: nothing before a pass-1 tile or before leaving,
+/// and a phase already seen complete once is not waited for again.
+CS_HD inline int must_see(const Leader& L, int p) {
+    const int q = p == 1 ? 0 : p == 2 ? 1 : -1;
+    return q >= 0 && ((L.seen >> q) & 1u) == 0u ? q : -1;
+}
+
+/// After running a unit of phase p.
+CS_HD inline void ran(Leader& L, int p) {
+    if (p < 2) ++L.held[p];
+}
+}  // namespace queue
+
 CS_COMPONENTS_BEGIN
 
-/// Per-instance counts, [6][K] column-major. params = {R, RT, G}.
+/// Per-instance counts, [9][K] column-major. params = {R, RT, G}; rank = each instance's place in the work order.
 class BatchPlan : public cs::Component<BatchPlan> {
 public:
     struct Inputs {
         cs::Input<DType::u32, 1> sizes;
         cs::Input<DType::u32, 1> params;
+        cs::Input<DType::u32, 1> rank;
     };
     struct Outputs {
         cs::Output<DType::u32, 1> counts;
@@ -341,9 +435,12 @@ public:
         out.counts[0 * K + k] = n;
         out.counts[1 * K + k] = n * ld_of(n) / 4u;
         out.counts[2 * K + k] = st * n;
-        out.counts[3 * K + k] = st * ceil_div(n, kTile1);
-        out.counts[4 * K + k] = ceil_div(n, kCols);
-        out.counts[5 * K + k] = ceil_div(n, RT);
+        const std::uint32_t u[3] = {st * ceil_div(n, kTile1), ceil_div(n, kCols), ceil_div(n, RT)};
+        const std::size_t   r    = in.rank[k];
+        for (int p = 0; p < 3; ++p) {
+            out.counts[(3 + p) * K + k] = u[p];
+            if (r < K) out.counts[(6 + p) * K + r] = u[p];  // the host passes a permutation of 0 .. K-1
+        }
     }
 
     Inputs  inputs;
@@ -351,9 +448,13 @@ public:
     void    declare() override {
         declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.params, "params", 3);
+        declare_input(inputs.rank, "rank", cs::unspecified_shape<1>());
         declare_output(outputs.counts, "counts", cs::unspecified_shape<1>());
     }
-    void prepare_forward() override { set_shape(outputs.counts, 6 * shape_of(inputs.sizes)[0]); }
+    void prepare_forward() override {
+        if (shape_of(inputs.rank)[0] != shape_of(inputs.sizes)[0]) throw std::invalid_argument("batch: rank must hold one place per instance");
+        set_shape(outputs.counts, kColumns * shape_of(inputs.sizes)[0]);
+    }
     std::unique_ptr<cs::LaunchDescription> define_forward() override { return create_launch_config(cs::range_desc(inputs.sizes.shape())); }
 };
 
@@ -392,97 +493,113 @@ public:
     }
 };
 
-/// The persistent work-queue kernel.
+/// The persistent work-queue kernel: every unit of the three phases in ONE launch of G work-groups (the queue protocol
+/// above). Units come from the unit map, as in the phased form.
 class RaggedGemver : public cs::Component<RaggedGemver> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
-        cs::Input<DType::u32, 1> sizes, params, offs;
+        cs::Input<DType::u32, 1> sizes, params, offs, map1, mapR, map2;
         cs::Input<DType::f32, 1> A, u1, v1, u2, v2, y, z;
         cs::Input<DType::f32, 1> alpha, beta;
     };
     struct Outputs {
         cs::Output<DType::f32, 1> P, x, w;
-        cs::Output<DType::u32, 1> state;  ///< [ticket, finished, pass-1 done per instance, reductions done per instance]
+        cs::Output<DType::u32, 1> state;  ///< ticket, finished, pass-1 tiles published, reductions published (queue::)
     };
 
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Arrays         a     = arrays(in, out);
-        const Cols           cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
-        const std::uint32_t* sizes = cs::raw::data(in.sizes);
-        const std::uint32_t  K     = cols.K;
-        const std::uint32_t RT   = in.params[std::size_t{1}];
-        const std::uint32_t T1 = cols.total(3), TR = cols.total(4), T2 = cols.total(5), T = T1 + TR + T2;
-        std::uint32_t*      st = cs::raw::data(out.state);
-        std::uint32_t*      p1_done  = st + 2;
-        std::uint32_t*      red_done = st + 2 + K;
-        const std::uint32_t tid      = static_cast<std::uint32_t>(it.get_local_id(0));
+        const Arrays        a = arrays(in, out);
+        const Cols          cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const queue::Totals n{cols.total(3), cols.total(4), cols.total(5)};
+        const std::uint32_t T   = n.all();
+        const std::uint32_t RT  = in.params[std::size_t{1}];
+        std::uint32_t*      st  = cs::raw::data(out.state);
+        const std::uint32_t tid = static_cast<std::uint32_t>(it.get_local_id(0));
         __shared__ std::uint32_t s_unit;
 
-        // One leader-only block per iteration, between two barriers: it publishes the completion of the unit the group
-        // has just run (when a later unit waits on it: release fence, then the counter) and takes the next ticket. The only
-        // other leader-only block of the loop, a wait's poll, also sits between barriers, so no two of them can be merged
-        // across the back edge (such a merged pair once parked the leader in a nested loop while the other lanes re-read
-        // the previous unit). A work-group takes each unit at most once, so T + 1 fetches bound the loop whatever the
-        // state holds.
-        std::uint32_t* pending = nullptr;  // the completion counter of the unit just run (the same in every work-item)
-        for (std::uint32_t fetches = 0; fetches <= T; ++fetches) {
-            __syncthreads();  // every work-item's writes of the previous unit are issued, and every work-item has read s_unit
+        // Leader-only blocks sit between barriers: the first fetch; per iteration the publish / wait / next fetch before
+        // the unit and the hand-over of the fetched ticket after it (so no two of them can be merged across the loop's
+        // back edge), and the exit is decided from LDS after a barrier (uniform). The next ticket is fetched after the
+        // wait and before the unit, so its latency hides under the unit, and a work-group never waits while it holds an
+        // unstarted ticket of an earlier phase. A work-group takes each unit at most once: T + 1 fetches bound the loop.
+        queue::Leader L;       // meaningful in the leader only
+        std::uint32_t next = 0;
+        if (tid == 0) s_unit = __hip_atomic_fetch_add(&st[queue::kTicket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        std::uint32_t u       = detail::uniform(s_unit);
+        bool          drained = false;
+        for (std::uint32_t fetches = 1; fetches <= T + 1; ++fetches) {
+            const int p = queue::phase_of(u, n);
             if (tid == 0) {
-                if (pending != nullptr) {
+                std::uint32_t add[2];
+                if (queue::to_publish(L, p, add)) {
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    __hip_atomic_fetch_add(pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (int q = 0; q < 2; ++q)
+                        if (add[q] != 0u) __hip_atomic_fetch_add(&st[queue::done(q)], add[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
-                s_unit = __hip_atomic_fetch_add(&st[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int q = queue::must_see(L, p);
+                if (q >= 0) {
+                    const std::uint32_t need = n.of(q);
+                    if (!cs::bounded::poll([&] { return cs::detail::poll_load(&st[queue::done(q)]) >= need; }, it.spin_limit()))
+                        it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    L.seen |= 1u << q;
+                }
+                if (p != 3) next = __hip_atomic_fetch_add(&st[queue::kTicket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (p == 3) {
+                drained = true;
+                break;
+            }
+            __syncthreads();  // the leader's wait (and acquire) is behind every work-item
+            run_unit(a, in, p, u - n.base(p), n, RT, tid);
+            if (tid == 0) {
+                queue::ran(L, p);
+                s_unit = next;
             }
             __syncthreads();
-            pending             = nullptr;
-            const std::uint32_t u = s_unit;
-            if (u >= T) break;
-            if (u < T1) {
-                const std::uint32_t k = cols.find(3, u);
-                unit_pass1(a, cols.unit(sizes, 3, k, u), tid);
-                pending = &p1_done[k];
-            } else if (u < T1 + TR) {
-                const std::uint32_t k = cols.find(4, u - T1);
-                wait_for(it, &p1_done[k], cols.s[3 * K + k + 1] - cols.s[3 * K + k], tid);
-                unit_reduce(a, cols.unit(sizes, 4, k, u - T1), tid);
-                pending = &red_done[k];
-            } else {
-                const std::uint32_t k = cols.find(5, u - T1 - TR);
-                wait_for(it, &red_done[k], cols.s[4 * K + k + 1] - cols.s[4 * K + k], tid);
-                const Unit w = cols.unit(sizes, 5, k, u - T1 - TR);
-                if (RT == 4)
-                    unit_pass2<4>(a, w, tid);
-                else
-                    unit_pass2<8>(a, w, tid);
-            }
+            u = detail::uniform(s_unit);
         }
-        // leaving (a completion still pending only if the fetch bound ended the loop, which a zeroed state never does): the
-        // last work-group to leave zeroes the counters (every unit is done and nobody reads them any more)
+        // leaving: every unit this group ran is published (the p = 3 step above); the last work-group to leave zeroes the
+        // state (every unit is done and nobody reads it any more). A loop ended by the fetch bound found a state that was
+        // not zero at the launch: it publishes what it holds (so nobody waits for it) and reports.
         __syncthreads();
         __shared__ std::uint32_t s_last;
         if (tid == 0) {
-            if (pending != nullptr) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                __hip_atomic_fetch_add(pending, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!drained) {
+                std::uint32_t add[2];
+                if (queue::to_publish(L, 3, add)) {
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                    for (int q = 0; q < 2; ++q)
+                        if (add[q] != 0u) __hip_atomic_fetch_add(&st[queue::done(q)], add[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                it.report_fault(CS_HIP_FAULT_STATE);
             }
-            const std::uint32_t f = __hip_atomic_fetch_add(&st[1], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            const std::uint32_t f = __hip_atomic_fetch_add(&st[queue::kFinished], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             s_last                = f + 1 == static_cast<std::uint32_t>(it.get_group_range(0)) ? 1u : 0u;
         }
         __syncthreads();
-        if (s_last)
-            for (std::uint32_t w = tid; w < 2 + 2 * K; w += kWg) __hip_atomic_store(&st[w], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_last && tid < 4) __hip_atomic_store(&st[tid * queue::kStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 
-    // ---- the units ------------------------------------------------------------------------------------------------
-    /// Before a unit: the leader waits (acquire) for `target` completions of earlier units, at most the launch's spin limit
-    /// of polls (one that gives up reports CS_HIP_FAULT_UNIT_WAIT and the unit runs on stale data), then the group.
-    __device__ static void wait_for(const cs::GroupItem<1>& it, std::uint32_t* counter, std::uint32_t target, std::uint32_t tid) {
-        if (tid == 0 && !cs::bounded::poll([&] { return cs::detail::poll_load(counter) >= target; }, it.spin_limit()))
-            it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    /// Unit i of phase p, described by its unit-map entry (a map that cannot hold the phase's count runs nothing: the
+    /// host sizes the maps for every unit count it allows).
+    __device__ static void run_unit(const Arrays& a, const Inputs& in, int p, std::uint32_t i, const queue::Totals& n, std::uint32_t RT,
+                                    std::uint32_t tid) {
+        Unit w;
+        if (p == 0) {
+            if (map_unit(cs::raw::data(in.map1), in.map1.size(), i, n.T1, w)) unit_pass1(a, w, tid);
+        } else if (p == 1) {
+            if (map_unit(cs::raw::data(in.mapR), in.mapR.size(), i, n.TR, w)) unit_reduce(a, w, tid);
+        } else if (map_unit(cs::raw::data(in.map2), in.map2.size(), i, n.T2, w)) {
+            switch (RT) {
+                case 4: unit_pass2<4>(a, w, tid); break;
+                case 8: unit_pass2<8>(a, w, tid); break;
+                case 16: unit_pass2_waves<4>(a, w, tid); break;
+                default: unit_pass2_waves<8>(a, w, tid); break;  // 32: the host accepts no other value
+            }
+        }
     }
 
     __device__ static Arrays arrays(const Inputs& in, const Outputs& out) {
@@ -512,6 +629,9 @@ public:
         declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.params, "params", 3);
         declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.map1, "map1", cs::unspecified_shape<1>());
+        declare_input(inputs.mapR, "mapR", cs::unspecified_shape<1>());
+        declare_input(inputs.map2, "map2", cs::unspecified_shape<1>());
         for (auto [w, name] : {std::pair{&inputs.A, "A"}, {&inputs.u1, "u1"}, {&inputs.v1, "v1"}, {&inputs.u2, "u2"}, {&inputs.v2, "v2"},
                                {&inputs.y, "y"}, {&inputs.z, "z"}})
             declare_input(*w, name, cs::unspecified_shape<1>());
@@ -520,16 +640,20 @@ public:
         declare_output(outputs.P, "P", cs::unspecified_shape<1>());
         declare_output(outputs.x, "x", cs::unspecified_shape<1>());
         declare_output(outputs.w, "w", cs::unspecified_shape<1>());
-        declare_scratch(outputs.state, "state", cs::unspecified_shape<1>());
+        declare_scratch(outputs.state, "state", queue::kWords);
     }
     void prepare_forward() override {
-        const std::size_t K = shape_of(inputs.sizes)[0];
         set_shape(outputs.x, shape_of(inputs.u1)[0]);
         set_shape(outputs.w, shape_of(inputs.u1)[0]);
-        set_shape(outputs.state, 2 + 2 * K);
     }
-    /// G persistent work-groups (the grid): read from the knobs the owner shares (the autotuner's third knob)
+    /// G persistent work-groups (the grid, the autotuner's third knob); every phase's unit count must fit its map.
     std::unique_ptr<cs::LaunchDescription> define_forward() override {
+        const std::size_t maps[3] = {inputs.map1.shape()[0], inputs.mapR.shape()[0], inputs.map2.shape()[0]};
+        const char*       who[3]  = {"pass 1", "column reduction", "pass 2"};
+        for (int p = 0; p < 3; ++p)
+            if (knobs_->units[p] * kMapWords > maps[p])
+                throw std::logic_error(std::string{"batch: the "} + who[p] + " units (" + std::to_string(knobs_->units[p]) +
+                                       ") do not fit their unit map (" + std::to_string(maps[p] / kMapWords) + " entries)");
         return create_launch_config(cs::range_desc(cs::range<1>{static_cast<std::size_t>(knobs_->groups) * kWg}), cs::wg_desc(cs::range<1>{kWg}));
     }
     explicit RaggedGemver(std::shared_ptr<Knobs> k) : knobs_{std::move(k)} {}
@@ -540,13 +664,14 @@ private:
 
 // ---- the phased form: the unit map, then one plain launch per phase, work-group u runs unit u ----------------------
 
-/// For every unit of the three phases, its Unit at its scan position (map1: pass-1 tiles, mapR: column-reduction blocks,
-/// map2: pass-2 row groups; kMapWords u32 per entry: n, t, vo, a4, poff, k, 0, 0). One work-group per instance.
+/// For every unit of the three phases, its Unit at its place in the work order (map1: pass-1 tiles, mapR: column-reduction
+/// blocks, map2: pass-2 row groups; kMapWords u32 per entry: n, t, vo, a4, poff, k, 0, 0): instance k's units of phase p
+/// start at offset rank_k of column 6 + p. One work-group per instance.
 class UnitMap : public cs::Component<UnitMap> {
 public:
     static constexpr int max_work_group_size = kWg;
     struct Inputs {
-        cs::Input<DType::u32, 1> sizes, offs;
+        cs::Input<DType::u32, 1> sizes, offs, rank;
     };
     struct Outputs {
         cs::Output<DType::u32, 1> map1, mapR, map2;
@@ -560,8 +685,10 @@ public:
         const std::uint32_t n = in.sizes[std::size_t{k}], vo = cols.off(0, k), a4 = cols.off(1, k), poff = cols.off(2, k);
         std::uint32_t* const maps[3] = {cs::raw::data(out.map1), cs::raw::data(out.mapR), cs::raw::data(out.map2)};
         const std::size_t    caps[3] = {out.map1.size() / kMapWords, out.mapR.size() / kMapWords, out.map2.size() / kMapWords};
+        const std::uint32_t  r       = in.rank[std::size_t{k}];
+        if (r >= cols.K) return;  // not a permutation: write nothing
         for (int p = 0; p < 3; ++p) {
-            const std::uint32_t o = cols.off(3 + p, k), cnt = cols.count(3 + p, k);
+            const std::uint32_t o = cols.off(6 + p, r), cnt = cols.count(3 + p, k);
             if (static_cast<std::size_t>(o) + cnt > caps[p]) continue;  // a scan that disagrees with the plan: write nothing
             U4* dst = reinterpret_cast<U4*>(maps[p]) + 2 * static_cast<std::size_t>(o);
             for (std::uint32_t i = tid; i < cnt; i += kWg) {
@@ -576,6 +703,7 @@ public:
     void    declare() override {
         declare_input(inputs.sizes, "sizes", cs::unspecified_shape<1>());
         declare_input(inputs.offs, "offs", cs::unspecified_shape<1>());
+        declare_input(inputs.rank, "rank", cs::unspecified_shape<1>());
         declare_output(outputs.map1, "map1", cs::unspecified_shape<1>());
         declare_output(outputs.mapR, "mapR", cs::unspecified_shape<1>());
         declare_output(outputs.map2, "map2", cs::unspecified_shape<1>());
@@ -605,7 +733,7 @@ public:
         cs::Output<DType::f32, 1> P;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / kColumns)};
         Unit       w;
         if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(3), w)) return;
         Arrays a;
@@ -653,7 +781,7 @@ public:
         cs::Output<DType::f32, 1> x;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / kColumns)};
         Unit       w;
         if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(4), w)) return;
         Arrays a;
@@ -700,7 +828,7 @@ public:
         cs::Output<DType::f32, 1> w;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / 6)};
+        const Cols cols{cs::raw::data(in.offs), static_cast<std::uint32_t>((in.offs.size() - 1) / kColumns)};
         Unit       w;
         if (!map_unit(cs::raw::data(in.map), in.map.size(), static_cast<std::uint32_t>(it.get_group(0)), cols.total(5), w)) return;
         Arrays a;

@@ -171,13 +171,13 @@ int cs_chain_destroy(cs_chain c);
  * cospanmdao/include/naga/cospan/fused_component.hpp:207-213); the offsets come from the column build it uses for
  * ragged shapes (shape_hook.hpp:215-348). `count` instances with N_k = round(lo (hi/lo)^U_k) (U_k from the hash (seed, k));
  * each x_k = beta Â_k^T y_k + z_k, w_k = alpha Â_k x_k as GEMVER above. One forward = a plan kernel, ONE device scan of
- * the per-instance counts, then (phased form, the default) a unit map and one launch per phase over every instance:
- * pass 1, the column reductions, pass 2 -- or (persistent form) ONE work-queue launch; runs from the second on replay one
- * captured graph.
- * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (6 x count), "offs" (6 x count + 1: the scan),
+ * the per-instance counts, a unit map (every phase's units in the work order: largest instances first), then (phased
+ * form, the default) one launch per phase over every instance: pass 1, the column reductions, pass 2 -- or (persistent
+ * form) ONE work-queue launch that runs every unit of the three phases; runs from the second on replay one captured graph.
+ * Names for cs_batch_read (u32 or f32 words): "sizes", "counts" (9 x count), "offs" (9 x count + 1: the scan),
  * "result" ([total, flag] of the scan), "x", "w", "u1" .. "z" (instances back to back).
- * params: R (strip rows, >= 32), RT (pass-2 rows per work-group: 4 or 8; the phased form also 16 or 32, a wave per RT/4
- * rows), G (persistent work-groups; the phased form ignores it).
+ * params: R (strip rows, >= 32), RT (pass-2 rows per work-group: 4 or 8, the work-group shares a row tile; 16 or 32, a
+ * wave per RT/4 rows), G (persistent work-groups; the phased form ignores it).
  */
 typedef struct cs_batch_s* cs_batch;
 
@@ -187,8 +187,8 @@ int cs_batch_set_params(cs_batch b, uint32_t R, uint32_t RT, uint32_t groups);
 int cs_batch_get_params(cs_batch b, uint32_t* params3);
 /* 0: the phased form (default), 1: the persistent work-queue form; a change rebuilds the model at its next use */
 int cs_batch_set_mode(cs_batch b, int persistent);
-/* times every candidate (phased: R in {64, 128, 256}, RT in {8, 16, 32}; persistent: R in {32, 64, 128}, RT in {4, 8},
- * G in {2, 4, 8} x CUs) on this model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
+/* times every candidate (R in {64, 128, 256}, RT in {8, 16, 32}; persistent form also G in {2, 4, 8} x CUs) on this
+ * model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
  * read first and appended to */
 int cs_batch_autotune(cs_batch b, int reps, const char* cache_path, uint32_t* best3);
 int cs_batch_run(cs_batch b);
@@ -197,8 +197,8 @@ int cs_batch_read(cs_batch b, const char* name, void* host, size_t bytes);
 int cs_batch_time(cs_batch b, int iters, float* ms_per_run);
 /* `iters` back-to-back runs with a device event after each: ms[i] = the i-th run's time (for percentiles) */
 int cs_batch_time_each(cs_batch b, int iters, float* ms);
-/* one leaf of the model (0 plan, 1 scan, then 2 unit map, 3 pass 1, 4 column reductions, 5 pass 2 in the phased form,
- * or 2 the persistent kernel) run `iters` times back to back after the leaves before it: *ms = its average device time */
+/* one leaf of the model (0 plan, 1 scan, 2 unit map, then 3 pass 1, 4 column reductions, 5 pass 2 in the phased form,
+ * or 3 the persistent kernel) run `iters` times back to back after the leaves before it: *ms = its average device time */
 int cs_batch_time_stage(cs_batch b, int leaf, int iters, float* ms);
 int cs_batch_stats(cs_batch b, uint64_t* stats4);
 int cs_batch_destroy(cs_batch b);

@@ -7,6 +7,7 @@
 // offsets, the work-queue state zeroed at every start) and what the model computes (x bit for bit against the oracle's
 // strip partials + column reduction, w against the FP64 oracle), over eager runs, a capture, replays and parameter
 // changes, in both forms. Prints "OK" and exits 0 when every check holds.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -47,27 +48,39 @@ static void emulate_plan(const void* bytes, const std::uint32_t grid[3], const s
     const std::size_t K = a.units[2];
     CHECK(a.units[0] == 1 && a.units[1] == 1, "plan: units %zu %zu", (size_t)a.units[0], (size_t)a.units[1]);
     CHECK(static_cast<std::size_t>(grid[0]) * block[0] >= K, "plan: grid %u x %u does not cover %zu instances", grid[0], block[0], K);
-    CHECK(a.in.sizes.size() == K && a.out.counts.size() == 6 * K, "plan: sizes %zu counts %zu for K %zu", a.in.sizes.size(),
-          a.out.counts.size(), K);
+    CHECK(a.in.sizes.size() == K && a.in.rank.size() == K && a.out.counts.size() == batch::kColumns * K, "plan: sizes %zu rank %zu counts %zu for K %zu",
+          a.in.sizes.size(), a.in.rank.size(), a.out.counts.size(), K);
     CHECK(mock::is_device(cs::raw::data(a.in.sizes)) && mock::is_device(cs::raw::data(a.out.counts)), "plan: a port outside device memory");
     const std::uint32_t* sz = cs::raw::data(a.in.sizes);
     const std::uint32_t* pr = cs::raw::data(a.in.params);
+    const std::uint32_t* rk = cs::raw::data(a.in.rank);
     std::uint32_t*       c  = cs::raw::data(a.out.counts);
     const std::uint32_t  R = pr[0], RT = pr[1];
+    // the work order: the instances by size, largest first, ties in index order
+    std::vector<bool> seen(K, false);
+    for (std::size_t k = 0; k < K; ++k) {
+        CHECK(rk[k] < K && !seen[rk[k]], "plan: rank is not a permutation (instance %zu at %u)", k, rk[k]);
+        if (rk[k] < K) seen[rk[k]] = true;
+        for (std::size_t m = 0; m < K; ++m)
+            if (m != k && (sz[m] > sz[k] || (sz[m] == sz[k] && m < k))) CHECK(rk[m] < rk[k], "plan: instance %zu (n %u) ranks after %zu (n %u)", m, sz[m], k, sz[k]);
+    }
     for (std::size_t k = 0; k < K; ++k) {
         const std::uint32_t n = sz[k], st = (n + R - 1) / R, bl = (n + batch::kCols - 1) / batch::kCols;
+        const std::uint32_t u[3] = {st * ((n + batch::kTile1 - 1) / batch::kTile1), bl, (n + RT - 1) / RT};
         c[k]         = n;
         c[K + k]     = n * batch::ld_of(n) / 4;
         c[2 * K + k] = st * n;
-        c[3 * K + k] = st * ((n + batch::kTile1 - 1) / batch::kTile1);
-        c[4 * K + k] = bl;
-        c[5 * K + k] = (n + RT - 1) / RT;
+        for (int p = 0; p < 3; ++p) {
+            c[(3 + p) * K + k] = u[p];
+            if (rk[k] < K) c[(6 + p) * K + rk[k]] = u[p];
+        }
     }
 }
 
 // ---- the units, as the device runs them (same fma order) -------------------------------------------------------------
 struct Emu {
     const std::uint32_t *s = nullptr, *sz = nullptr;
+    std::vector<std::uint32_t> order;  ///< order[r] = the instance at place r of the work order
     std::uint32_t        K = 0, R = 0, RT = 0;
     const float *        A = nullptr, *U1 = nullptr, *V1 = nullptr, *U2 = nullptr, *V2 = nullptr, *Y = nullptr, *Z = nullptr;
     const float *        Pin = nullptr, *Xin = nullptr;
@@ -87,9 +100,16 @@ struct Emu {
         }
         return lo;
     }
+    /// The instance of unit u of phase p (its place in the phase's queue) and the unit's index t within the instance.
+    std::uint32_t locate(int p, std::uint32_t u, std::uint32_t& t) const {
+        const std::uint32_t r = find(6 + p, u);
+        t                     = u - off(6 + p, r);
+        return order[r];
+    }
     /// pass-1 unit u (global over the instances); returns its instance. A tile is R rows x kTile1 columns.
     std::uint32_t pass1(std::uint32_t u) const {
-        const std::uint32_t k = find(3, u), t = u - off(3, k), n = sz[k], ld = batch::ld_of(n);
+        std::uint32_t       t;
+        const std::uint32_t k = locate(0, u, t), n = sz[k], ld = batch::ld_of(n);
         const std::uint32_t blocks = (n + batch::kTile1 - 1) / batch::kTile1, c = t / blocks, b = t % blocks;
         const std::size_t   vo = off(0, k);
         const float*        Ak = A + 4ull * off(1, k);
@@ -104,7 +124,8 @@ struct Emu {
         return k;
     }
     std::uint32_t reduce(std::uint32_t u) const {
-        const std::uint32_t k = find(4, u), b = u - off(4, k), n = sz[k];
+        std::uint32_t       b;
+        const std::uint32_t k = locate(1, u, b), n = sz[k];
         const std::uint32_t strips = (n + R - 1) / R;
         const std::size_t   vo     = off(0, k);
         const float*        Pk     = Pin + off(2, k);
@@ -116,7 +137,8 @@ struct Emu {
         return k;
     }
     std::uint32_t pass2(std::uint32_t u) const {
-        const std::uint32_t k = find(5, u), g = u - off(5, k), n = sz[k], ld = batch::ld_of(n);
+        std::uint32_t       g;
+        const std::uint32_t k = locate(2, u, g), n = sz[k], ld = batch::ld_of(n);
         const std::size_t   vo = off(0, k);
         const float*        Ak = A + 4ull * off(1, k);
         for (std::uint32_t r = g * RT; r < std::min(n, g * RT + RT); ++r) {
@@ -134,6 +156,14 @@ struct Emu {
 static const std::uint32_t* g_sizes = nullptr;
 static std::uint32_t        g_K     = 0;
 
+/// The work order the host gives the model: the instances by size, largest first, ties in index order.
+static std::vector<std::uint32_t> work_order() {
+    std::vector<std::uint32_t> o(g_K);
+    for (std::uint32_t k = 0; k < g_K; ++k) o[k] = k;
+    std::stable_sort(o.begin(), o.end(), [](std::uint32_t a, std::uint32_t b) { return g_sizes[a] > g_sizes[b]; });
+    return o;
+}
+
 template <class In>
 static Emu emu_common(const In& in, const char* who) {
     Emu e;
@@ -142,41 +172,60 @@ static Emu emu_common(const In& in, const char* who) {
     e.sz = g_sizes;
     e.R  = cs::raw::data(in.params)[0];
     e.RT = cs::raw::data(in.params)[1];
-    CHECK(in.offs.size() == 6 * e.K + 1, "%s: offs holds %zu words, not %u", who, in.offs.size(), 6 * e.K + 1);
-    CHECK(std::size_t(e.s[6 * e.K]) <= (std::size_t(1) << 31), "%s: the scan total %u is implausible", who, e.s[6 * e.K]);
+    e.order = work_order();
+    CHECK(in.offs.size() == batch::kColumns * e.K + 1, "%s: offs holds %zu words, not %u", who, in.offs.size(), batch::kColumns * e.K + 1);
+    CHECK(std::size_t(e.s[batch::kColumns * e.K]) <= (std::size_t(1) << 31), "%s: the scan total %u is implausible", who, e.s[batch::kColumns * e.K]);
     return e;
 }
 
-/// The unit map's entry for unit u of a phase must name the unit the scan puts there.
-static void check_map(const Emu& e, const std::uint32_t* map, std::size_t map_elems, int c, std::uint32_t u, const char* who) {
+/// The unit map's entry for unit u of phase p must name the unit the work order puts there.
+static void check_map(const Emu& e, const std::uint32_t* map, std::size_t map_elems, int p, std::uint32_t u, const char* who) {
     CHECK((std::size_t(u) + 1) * batch::kMapWords <= map_elems, "%s: unit %u is outside its map (%zu words)", who, u, map_elems);
     if ((std::size_t(u) + 1) * batch::kMapWords > map_elems) return;
     const std::uint32_t* m = map + std::size_t(u) * batch::kMapWords;
-    const std::uint32_t  k = e.find(c, u);
-    const bool ok = m[0] == e.sz[k] && m[1] == u - e.off(c, k) && m[2] == e.off(0, k) && m[3] == e.off(1, k) && m[4] == e.off(2, k) && m[5] == k;
+    std::uint32_t        t;
+    const std::uint32_t  k = e.locate(p, u, t);
+    const bool ok = m[0] == e.sz[k] && m[1] == t && m[2] == e.off(0, k) && m[3] == e.off(1, k) && m[4] == e.off(2, k) && m[5] == k;
     CHECK(ok, "%s: map entry of unit %u reads n %u t %u vo %u a4 %u poff %u k %u; the scan says instance %u", who, u, m[0], m[1], m[2], m[3],
           m[4], m[5], k);
 }
 
-// ---- RaggedGemver: the persistent work queue, its units in ticket order ----------------------------------------------
+// ---- RaggedGemver: the persistent work queue, its protocol run by simulated work-groups -------------------------------
+// The kernel's queue decisions are batch::queue's host-and-device functions; here G simulated work-groups step through
+// them (first fetch; per unit: publish, wait, fetch the next ticket, run, hand over; leave) in an order a seeded generator
+// picks, with at most `resident` work-groups started at a time (the rest start only when one leaves: no co-residency).
+// A unit runs at once when its step is taken, so the checks are that every reduction of an instance comes after all of
+// its pass-1 tiles and every pass-2 group after all of its reductions, that a work-group never fetches more than T + 1
+// tickets, that the queue always drains (some work-group can step until all have left), and that it leaves the state
+// zeroed.
+static std::uint64_t g_rng = 0x9e3779b97f4a7c15ull;
+static std::uint32_t next_rand() {
+    g_rng ^= g_rng << 13;
+    g_rng ^= g_rng >> 7;
+    g_rng ^= g_rng << 17;
+    return static_cast<std::uint32_t>(g_rng >> 16);
+}
+static std::uint64_t g_queue_runs = 0, g_queue_waits = 0;
+
 static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const std::uint32_t block[3]) {
+    namespace q = batch::queue;
     ++g_gemver_launches;
     const auto&         a = static_cast<const GAGem*>(bytes)->args.head;
     const std::uint32_t G = static_cast<std::uint32_t>(a.units[2]);
     CHECK(grid[0] == G && grid[1] == 1 && grid[2] == 1 && block[0] == batch::kWg, "gemver: grid %u %u %u block %u for G %u", grid[0],
           grid[1], grid[2], block[0], G);
     const std::uint32_t K = static_cast<std::uint32_t>(a.in.sizes.size());
-    CHECK(a.out.state.size() == 2 + 2 * K, "gemver: state holds %zu words, not %u", a.out.state.size(), 2 + 2 * K);
-    std::uint32_t* st = cs::raw::data(a.out.state);
+    CHECK(a.out.state.size() == q::kWords, "gemver: state holds %zu words, not %d", a.out.state.size(), q::kWords);
+    std::uint32_t* st   = cs::raw::data(a.out.state);
     bool           zero = true;
-    for (std::uint32_t w = 0; w < 2 + 2 * K; ++w) zero = zero && st[w] == 0;
-    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u)",
-          (unsigned long long)g_gemver_launches, st[0], st[1]);
+    for (int w = 0; w < q::kWords; ++w) zero = zero && st[w] == 0;
+    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u, done %u %u)",
+          (unsigned long long)g_gemver_launches, st[q::kTicket], st[q::kFinished], st[q::done(0)], st[q::done(1)]);
     if (!zero) return;
 
     // the launch carries the device's fault word and the current poll bound; a bound of 0 makes every wait give up
-    const auto& geo   = static_cast<const GAGem*>(bytes)->geo;
-    std::uint32_t* fw = nullptr;
+    const auto&    geo = static_cast<const GAGem*>(bytes)->geo;
+    std::uint32_t* fw  = nullptr;
     cs_hip_fault_word(&fw);
     CHECK(geo.fault == fw, "gemver: the launch does not carry the fault word");
     CHECK(geo.spin_limit == cs_hip_spin_limit(), "gemver: the launch carries spin limit %u, the process %u", geo.spin_limit,
@@ -188,27 +237,112 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
     e.P = cs::raw::data(a.out.P), e.X = cs::raw::data(a.out.x), e.W = cs::raw::data(a.out.w);
     e.Pin = e.P, e.Xin = e.X;
     e.alpha = cs::raw::data(a.in.alpha)[0], e.beta = cs::raw::data(a.in.beta)[0];
-    const std::uint32_t T1 = e.total(3), TR = e.total(4), T2 = e.total(5);
+    const q::Totals n{e.total(3), e.total(4), e.total(5)};
+    const std::uint32_t T = n.all();
     CHECK(a.in.A.size() >= 4ull * e.total(1), "gemver: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
     CHECK(a.out.P.size() >= e.total(2), "gemver: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
     CHECK(a.out.x.size() >= e.total(0) && a.in.u1.size() >= e.total(0), "gemver: vectors shorter than the scan's total");
-    if (geo.spin_limit == 0 && TR + T2 > 0) *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // the waits give up without polling
+    const std::uint32_t* maps[3]     = {cs::raw::data(a.in.map1), cs::raw::data(a.in.mapR), cs::raw::data(a.in.map2)};
+    const std::size_t    map_el[3]   = {a.in.map1.size(), a.in.mapR.size(), a.in.map2.size()};
+    const bool           give_up     = geo.spin_limit == 0;
+
+    enum Where { kNotStarted, kTop, kRun, kLeft };
+    struct WG {
+        Where         at      = kNotStarted;
+        q::Leader     L;
+        std::uint32_t u = 0, next = 0, fetches = 0;
+    };
+    std::vector<WG>            wg(G);
     std::vector<std::uint32_t> p1(K, 0), red(K, 0);
-    for (std::uint32_t u = 0; u < T1 + TR + T2; ++u) {
-        if (u < T1) {
-            ++p1[e.pass1(u)];
-        } else if (u < T1 + TR) {
-            const std::uint32_t k = e.find(4, u - T1);
-            CHECK(p1[k] == e.s[3 * K + k + 1] - e.s[3 * K + k], "gemver: reduction of instance %u before its pass-1 tiles", k);
-            ++red[e.reduce(u - T1)];
-        } else {
-            const std::uint32_t k = e.find(5, u - T1 - TR);
-            CHECK(red[k] == e.s[4 * K + k + 1] - e.s[4 * K + k], "gemver: pass 2 of instance %u before its reductions", k);
-            e.pass2(u - T1 - TR);
+    const std::uint32_t        resident = 1 + next_rand() % std::max<std::uint32_t>(G, 1);
+    std::uint32_t              started = 0, left = 0;
+    auto fetch = [&](WG& w) {
+        ++w.fetches;
+        CHECK(w.fetches <= T + 1, "gemver: a work-group fetched %u tickets, the bound is T + 1 = %u", w.fetches, T + 1);
+        return st[q::kTicket]++;
+    };
+    // one step of work-group w; false when it cannot step now (not started while the device is full, or waiting)
+    auto step = [&](WG& w) -> bool {
+        switch (w.at) {
+            case kNotStarted:
+                if (started - left >= resident) return false;
+                ++started;
+                w.u  = fetch(w);
+                w.at = kTop;
+                return true;
+            case kTop: {
+                const int p = q::phase_of(w.u, n);
+                std::uint32_t add[2];
+                if (q::to_publish(w.L, p, add)) {
+                    st[q::done(0)] += add[0];
+                    st[q::done(1)] += add[1];
+                }
+                const int c = q::must_see(w.L, p);
+                if (c >= 0) {
+                    if (give_up) {
+                        *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // a poll bound of 0 gives up at once and runs on
+                    } else if (st[q::done(c)] < n.of(c)) {
+                        ++g_queue_waits;
+                        return false;
+                    }
+                    w.L.seen |= 1u << c;
+                }
+                if (p == 3) {
+                    w.at = kLeft;
+                    if (++st[q::kFinished] == G) std::memset(st, 0, q::kWords * 4);  // the last to leave zeroes the state
+                    ++left;
+                    return true;
+                }
+                w.next = fetch(w);
+                w.at   = kRun;
+                return true;
+            }
+            case kRun: {
+                const int           p = q::phase_of(w.u, n);
+                const std::uint32_t i = w.u - n.base(p);
+                check_map(e, maps[p], map_el[p], p, i, "gemver");
+                if (p == 0) {
+                    ++p1[e.pass1(i)];
+                } else if (p == 1) {
+                    std::uint32_t       t;
+                    const std::uint32_t k = e.locate(1, i, t);
+                    CHECK(give_up || p1[k] == e.s[3 * K + k + 1] - e.s[3 * K + k], "gemver: reduction of instance %u before its pass-1 tiles", k);
+                    ++red[e.reduce(i)];
+                } else {
+                    std::uint32_t       t;
+                    const std::uint32_t k = e.locate(2, i, t);
+                    CHECK(give_up || red[k] == e.s[4 * K + k + 1] - e.s[4 * K + k], "gemver: pass 2 of instance %u before its reductions", k);
+                    e.pass2(i);
+                }
+                ++g_queue_runs;
+                q::ran(w.L, p);
+                w.u  = w.next;
+                w.at = kTop;
+                return true;
+            }
+            default: return false;
+        }
+    };
+    std::vector<std::uint32_t> can;
+    while (left < G) {
+        can.clear();
+        for (std::uint32_t g = 0; g < G; ++g)
+            if (wg[g].at != kLeft) can.push_back(g);
+        // try the work-groups in a random order until one steps; none can: the queue is stuck
+        bool moved = false;
+        for (std::size_t tries = can.size(); tries > 0 && !moved; --tries) {
+            const std::size_t pick = next_rand() % can.size();
+            moved                  = step(wg[can[pick]]);
+            if (!moved) can.erase(can.begin() + static_cast<std::ptrdiff_t>(pick));
+        }
+        if (!moved) {
+            CHECK(false, "gemver: the work queue is stuck (G %u, resident %u, %u left, ticket %u, done %u/%u %u/%u)", G, resident, left,
+                  st[q::kTicket], st[q::done(0)], n.T1, st[q::done(1)], n.TR);
+            std::memset(st, 0, q::kWords * 4);
+            return;
         }
     }
-    // the last work-group to leave zeroes the state (it was zero, and the queue leaves it so)
-    std::memset(st, 0, (2 + 2 * K) * 4);
+    for (int w = 0; w < q::kWords; ++w) CHECK(st[w] == 0, "gemver: state word %d is %u after the launch", w, st[w]);
 }
 
 // ---- the phased form: one launch per phase, work-group u runs unit u; the grid must cover the device plan's count ----
@@ -230,13 +364,15 @@ static void emulate_map(const void* bytes, const std::uint32_t grid[3], const st
     e.K  = static_cast<std::uint32_t>(a.in.sizes.size());
     e.s  = cs::raw::data(a.in.offs);
     e.sz = cs::raw::data(a.in.sizes);
-    CHECK(e.K == g_K && a.in.offs.size() == 6 * e.K + 1, "map: %u instances, offs %zu", e.K, a.in.offs.size());
+    CHECK(e.K == g_K && a.in.offs.size() == batch::kColumns * e.K + 1 && a.in.rank.size() == e.K, "map: %u instances, offs %zu, rank %zu", e.K,
+          a.in.offs.size(), a.in.rank.size());
+    const std::uint32_t* rk = cs::raw::data(a.in.rank);
     CHECK(grid[0] == e.K && block[0] == batch::kWg, "map: grid %u block %u for %u instances", grid[0], block[0], e.K);
     std::uint32_t* maps[3] = {cs::raw::data(a.out.map1), cs::raw::data(a.out.mapR), cs::raw::data(a.out.map2)};
     const std::size_t caps[3] = {a.out.map1.size() / batch::kMapWords, a.out.mapR.size() / batch::kMapWords, a.out.map2.size() / batch::kMapWords};
     for (std::uint32_t k = 0; k < e.K; ++k)
         for (int p = 0; p < 3; ++p) {
-            const std::uint32_t o = e.off(3 + p, k), cnt = e.s[(3 + p) * e.K + k + 1] - e.s[(3 + p) * e.K + k];
+            const std::uint32_t o = e.off(6 + p, rk[k]), cnt = e.s[(3 + p) * e.K + k + 1] - e.s[(3 + p) * e.K + k];
             CHECK(std::size_t(o) + cnt <= caps[p], "map: phase %d of instance %u ends at %u, the map holds %zu", p, k, o + cnt, caps[p]);
             if (std::size_t(o) + cnt > caps[p]) continue;
             for (std::uint32_t i = 0; i < cnt; ++i) {
@@ -257,7 +393,7 @@ static void emulate_pass1(const void* bytes, const std::uint32_t grid[3], const 
     CHECK(a.out.P.size() >= e.total(2), "pass 1: P holds %zu floats, the scan asks %u", a.out.P.size(), e.total(2));
     CHECK(a.in.A.size() >= 4ull * e.total(1), "pass 1: A holds %zu floats, the scan asks %llu", a.in.A.size(), 4ull * e.total(1));
     for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(3)); ++u) {
-        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 3, u, "pass 1");
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 0, u, "pass 1");
         e.pass1(u);
     }
 }
@@ -270,7 +406,7 @@ static void emulate_reduce(const void* bytes, const std::uint32_t grid[3], const
     check_phase_grid(a, grid, block, e.total(4), "reduce");
     CHECK(a.out.x.size() >= e.total(0) && a.in.z.size() >= e.total(0), "reduce: vectors shorter than the scan's total");
     for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(4)); ++u) {
-        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 4, u, "reduce");
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 1, u, "reduce");
         e.reduce(u);
     }
 }
@@ -284,7 +420,7 @@ static void emulate_pass2(const void* bytes, const std::uint32_t grid[3], const 
     check_phase_grid(a, grid, block, e.total(5), "pass 2");
     CHECK(a.out.w.size() >= e.total(0) && a.in.x.size() >= e.total(0), "pass 2: vectors shorter than the scan's total");
     for (std::uint32_t u = 0; u < std::min<std::uint32_t>(static_cast<std::uint32_t>(a.units[2]), e.total(5)); ++u) {
-        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 5, u, "pass 2");
+        check_map(e, cs::raw::data(a.in.map), a.in.map.size(), 2, u, "pass 2");
         e.pass2(u);
     }
 }
@@ -386,13 +522,17 @@ static void run_form(bool phased, cs::ReplayMode mode = cs::ReplayMode::graph) {
     for (int i = 0; i < 3; ++i) m.run();
     std::snprintf(what, sizeof what, "%s: after a parameter change (R 64, RT 8, G 17)", form);
     check_outputs(m, sizes, 64, seed, alpha, beta, what);
-    if (phased) {  // wave-level pass 2 (RT 32) and long strips
-        params = {256, 32, 5};
-        batch::count_units(sizes.data(), count, 256, 32, knobs->units);
+    // wave-level pass 2 (RT 16, 32) and long strips; the persistent form over grids from one work-group to more than
+    // there are units, each run a new interleaving and residency cap
+    const std::uint32_t more[][3] = {{256, 32, 5}, {32, 16, 1}, {128, 32, 2}, {64, 16, 400}, {32, 32, 7}};
+    for (const auto& c : more) {
+        params        = {c[0], c[1], c[2]};
+        knobs->groups = static_cast<int>(c[2]);
+        batch::count_units(sizes.data(), count, c[0], c[1], knobs->units);
         m.update_input("params", cs::init::from_host(params));
-        for (int i = 0; i < 3; ++i) m.run();
-        std::snprintf(what, sizeof what, "%s: R 256, RT 32", form);
-        check_outputs(m, sizes, 256, seed, alpha, beta, what);
+        for (int i = 0; i < 4; ++i) m.run();
+        std::snprintf(what, sizeof what, "%s: R %u, RT %u, G %u", form, c[0], c[1], c[2]);
+        check_outputs(m, sizes, c[0], seed, alpha, beta, what);
     }
 }
 
@@ -457,6 +597,8 @@ int main() {
     const std::string f = mock::failure();
     CHECK(f.empty(), "mock: %s", f.c_str());
     CHECK(g_gemver_launches >= 4, "the persistent kernel ran %llu times", (unsigned long long)g_gemver_launches);
+    CHECK(g_queue_waits > 0, "no simulated work-group ever had to wait (the interleavings never tested a wait)");
+    std::printf("persistent queue: %llu units run, %llu blocked steps\n", (unsigned long long)g_queue_runs, (unsigned long long)g_queue_waits);
     for (int p = 0; p < 3; ++p) CHECK(g_phase_launches[p] >= 4, "phase %d ran %llu times", p, (unsigned long long)g_phase_launches[p]);
     CHECK(g_map_launches >= 4, "the unit map ran %llu times", (unsigned long long)g_map_launches);
     if (g_failures) {
