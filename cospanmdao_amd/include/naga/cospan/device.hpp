@@ -158,10 +158,16 @@ __device__ inline T reduce_first(T x, Op op, int count) {
     if (nthreads <= wave::kSize) return x;   // one wave: reduce_first already broadcast lane 0's result
     if ((tid & (wave::kSize - 1)) == 0 && w < nwaves) slots[w] = x;
     __syncthreads();
+    // the tree over the wave results, unrolled over the 16-wave bound so that v stays in registers (a loop bounded by
+    // nwaves would index v at run time, which puts it in scratch memory: per work-item stores and loads to HBM)
     T v[16];
-    for (int k = 0; k < nwaves; ++k) v[k] = slots[k];
-    for (int width = 1; width < nwaves; width <<= 1)
-        for (int k = 0; k + width < nwaves; k += 2 * width) v[k] = op(v[k], v[k + width]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = slots[k < nwaves ? k : 0];
+#pragma unroll
+    for (int width = 1; width < 16; width <<= 1)
+#pragma unroll
+        for (int k = 0; k + width < 16; k += 2 * width)
+            if (k + width < nwaves) v[k] = op(v[k], v[k + width]);
     __syncthreads();
     return v[0];
 }

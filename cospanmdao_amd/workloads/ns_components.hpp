@@ -22,10 +22,12 @@
 // registers (i - 1, i, i + 1 in use, i + 2 in flight, i + 3 loading while row i is computed), so every field element is
 // read from HBM once (plus two halo rows per tile) and a row's loads have two iterations to arrive. The x neighbours of a work-item's V columns come from the adjacent
 // lanes by DPP wave shifts; only lanes 0 and 63 load their halo column from memory. Tiles are mapped to XCDs in
-// contiguous runs, so the halo rows of consecutive tiles meet in the same L2. S is the storage of u, v, p: f32 or bf16.
+// contiguous runs and sweep down in even strips, up in odd ones, so the two tiles on a strip boundary read the rows they
+// share at the same time, from the same L2. S is the storage of u, v, p: f32 or bf16.
 
 #include <cstdint>
 #include <memory>
+#include <type_traits>
 
 #include "naga/cospan/cospan.hpp"
 
@@ -178,10 +180,8 @@ public:
             h.vr = F::one(V[off + jr]);
             h.pr = F::one(P[off + jr]);
         };
-        auto wrap = [&](std::uint32_t i) { return i >= ny ? i - ny : i; };
         double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        // Row i of the tile from the window m (i - 1), c (i), n (i + 1) and row i's halo: the residual stored, its
-        // squares summed.
+        // Row i of the tile from the rows i - 1, i, i + 1 and row i's halo: the residual stored, its squares summed.
         auto emit = [&](std::uint32_t i, const Row& wm, const Row& wc, const Row& wn, const Halo& h) __attribute__((always_inline)) {
             float mu[kVec], mv[kVec], mp[kVec], cu[kVec], cv[kVec], cp[kVec], nu_[kVec], nv[kVec], np[kVec];
             F::widen(wm.u, mu);
@@ -234,44 +234,68 @@ public:
                 }
             }
         };
-        // A ring of five rows: while row i is computed from rows i - 1, i, i + 1, row i + 2 is in flight and row i + 3
-        // starts loading, so each row has two iterations to arrive. The halo of row i + 1 is issued one iteration ahead,
-        // BEFORE row i + 3: loads complete in order, so waiting for the halo never waits for the newer row.
-        // Invariant at the top of row i: r0 = row i - 1, r1 = row i, r2 = row i + 1, r3 = row i + 2, hn = halo of row i.
-        Row  r0{}, r1{}, r2{}, r3{}, r4{};
-        Halo hc{}, hn{};
-        load_row(i0 == 0 ? ny - 1 : i0 - 1, r0);
-        load_row(i0, r1);
-        load_row(wrap(i0 + 1), r2);
-        load_halo(i0, hn);
-        if (i0 + 1 < i1) load_row(wrap(i0 + 2), r3);
-        // Steady state: five rows per pass, every load in range, no exit inside (the wait counts stay exact). The loop is
-        // unrolled by the ring's length, so the rows never move between registers.
-        auto step = [&](std::uint32_t i, const Row& m, const Row& c, const Row& n, Row& next) __attribute__((always_inline)) {
-            hc = hn;
-            load_halo(i + 1, hn);
-            load_row(wrap(i + 3), next);  // row i + 3 is the middle row two iterations from now
-            emit(i, m, c, n, hc);
+        // The sweep over the tile's rows, top-down (kDown) or bottom-up. Tiles in even strips sweep down and tiles in odd
+        // strips up, so the two tiles on either side of a strip boundary reach the rows they share (each one's edge row is
+        // the other's halo row) at the same time, while those rows are in their XCD's L2: the halo rows are read from HBM
+        // once, not twice. Position s = 0 .. len - 1 is the s-th row swept; s = -1 and s = len are the halo rows behind
+        // and ahead of the sweep. A ring of five rows: while row s is computed from rows s - 1, s, s + 1, row s + 2 is in
+        // flight and row s + 3 starts loading, so each row has two iterations to arrive. The halo columns of row s + 1
+        // are issued one iteration ahead, BEFORE row s + 3: loads complete in order, so waiting for the halo never waits
+        // for the newer row. Invariant at the top of position s: r0 = s - 1, r1 = s, r2 = s + 1, r3 = s + 2, hn = halo
+        // columns of s.
+        auto sweep = [&](auto down_tag) __attribute__((always_inline)) {
+            constexpr bool     kDown = decltype(down_tag)::value;
+            const int          len   = static_cast<int>(i1 - i0);
+            auto               at    = [&](int q) {
+                const std::int64_t v = kDown ? static_cast<std::int64_t>(i0) + q : static_cast<std::int64_t>(i1) - 1 - q;
+                return static_cast<std::uint32_t>(v < 0 ? v + ny : v >= static_cast<std::int64_t>(ny) ? v - ny : v);
+            };
+            // the row behind the sweep is i - 1 going down and i + 1 going up
+            auto put = [&](int q, const Row& behind, const Row& c, const Row& ahead, const Halo& h) __attribute__((always_inline)) {
+                if constexpr (kDown)
+                    emit(at(q), behind, c, ahead, h);
+                else
+                    emit(at(q), ahead, c, behind, h);
+            };
+            Row  r0{}, r1{}, r2{}, r3{}, r4{};
+            Halo hc{}, hn{};
+            load_row(at(-1), r0);
+            load_row(at(0), r1);
+            load_row(at(1), r2);
+            load_halo(at(0), hn);
+            if (len > 1) load_row(at(2), r3);
+            // Steady state: five rows per pass, every load in range, no exit inside (the wait counts stay exact). The loop
+            // is unrolled by the ring's length, so the rows never move between registers.
+            auto step = [&](int q, const Row& m, const Row& c, const Row& n, Row& next) __attribute__((always_inline)) {
+                hc = hn;
+                load_halo(at(q + 1), hn);
+                load_row(at(q + 3), next);  // the middle row two iterations from now
+                put(q, m, c, n, hc);
+            };
+            int q = 0;
+            for (; q + 7 <= len; q += 5) {
+                step(q, r0, r1, r2, r4);
+                step(q + 1, r1, r2, r3, r0);
+                step(q + 2, r2, r3, r4, r1);
+                step(q + 3, r3, r4, r0, r2);
+                step(q + 4, r4, r0, r1, r3);
+            }
+            // The last rows (at most six): the ring shifts by register moves.
+            for (; q < len; ++q) {
+                hc = hn;
+                if (q + 1 < len) load_halo(at(q + 1), hn);
+                if (q + 2 < len) load_row(at(q + 3), r4);
+                put(q, r0, r1, r2, hc);
+                r0 = r1;
+                r1 = r2;
+                r2 = r3;
+                r3 = r4;
+            }
         };
-        std::uint32_t i = i0;
-        for (; i + 7 <= i1; i += 5) {
-            step(i, r0, r1, r2, r4);
-            step(i + 1, r1, r2, r3, r0);
-            step(i + 2, r2, r3, r4, r1);
-            step(i + 3, r3, r4, r0, r2);
-            step(i + 4, r4, r0, r1, r3);
-        }
-        // The last rows (at most six): the ring shifts by register moves.
-        for (; i < i1; ++i) {
-            hc = hn;
-            if (i + 1 < i1) load_halo(i + 1, hn);
-            if (i + 2 < i1) load_row(wrap(i + 3), r4);
-            emit(i, r0, r1, r2, hc);
-            r0 = r1;
-            r1 = r2;
-            r2 = r3;
-            r3 = r4;
-        }
+        if ((r & 1u) == 0u)
+            sweep(std::true_type{});
+        else
+            sweep(std::false_type{});
         // the tile's partial: a balanced tree over the work-group (deterministic)
         const auto g = it.get_group();
         s0           = cs::reduce_over_group(g, s0, cs::plus<double>());
