@@ -412,6 +412,19 @@ __device__ inline void load_f32(const float* p, float (&v)[N]) {
     }
 }
 
+/// The order in which a row sweep of pass 2 takes its `count` full column steps: starting at step `first` and wrapping
+/// around. Rows of A lie 4 n bytes apart, so sweeps that all start at column 0 and advance together keep every concurrent
+/// row stream at the same column offset, on the same few HBM channels; each wave (or work-group) starts at a step of its
+/// own instead (pass 2 at N = 16384: 157.5 -> 153.5 us).
+struct Rotation {
+    std::uint32_t          first = 0, count = 0;
+    __device__ static Rotation of(std::uint32_t who, std::uint32_t count) { return Rotation{count ? who * 5u % count : 0u, count}; }
+    __device__ std::uint32_t at(std::uint32_t t) const {
+        const std::uint32_t s = first + t;
+        return s >= count ? s - count : s;
+    }
+};
+
 // =============================================================================
 // Pass 1: P[c][j] = sum_{k<R} Â[cR + k][j] y[cR + k]
 // =============================================================================
@@ -695,6 +708,8 @@ public:
     /// 64 lanes x kVec columns per step, then reduces each row within the wave: no LDS, no barrier). Measured faster than
     /// the block layout below for the f32 headline (155.5 against 161 us per 1 GiB pass at 16 rows).
     static constexpr int kWaveRows = RT >= 16 ? RT / 4 : 0;
+    /// f32 storage in the wave layout: the staged step (wave_rows)
+    static constexpr bool kStaged = S == DType::f32 && kWaveRows > 0;
 
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         if constexpr (kWaveRows > 0) {
@@ -744,12 +759,15 @@ public:
             D1 = cs::reduce_over_group(it.get_group(), p1, cs::plus<float>());
             D2 = cs::reduce_over_group(it.get_group(), p2, cs::plus<float>());
         }
-        const std::uint32_t step = kWg * kVec;
-        std::uint32_t       j0   = tid * kVec;
-        // chunks unrolled so that 16 loads of 16 bytes of A are in flight per work-item
+        // the full column steps in this work-group's rotated order (chunks unrolled so that 16 loads of 16 bytes of A are
+        // in flight per work-item); then the work-item's columns of a partial last step
+        const std::uint32_t step  = kWg * kVec;
+        const std::uint32_t nfull = n / step;
+        const Rotation      rot   = Rotation::of(r0 / RT, nfull);
 #pragma unroll kChunksInFlight
-        for (; j0 + kVec <= n; j0 += step) {
-            float v1[kVec], v2[kVec], x[kVec];
+        for (std::uint32_t t = 0; t < nfull; ++t) {
+            const std::uint32_t j0 = rot.at(t) * step + tid * kVec;
+            float               v1[kVec], v2[kVec], x[kVec];
             if constexpr (!kFactored) {
                 load_f32<kVec>(v1p + j0, v1);
                 load_f32<kVec>(v2p + j0, v2);
@@ -791,18 +809,20 @@ public:
                 }
             }
         }
-        if (j0 < n && kFactored) {  // the ragged tail of the rows (this work-item's last, partial chunk)
+        const std::uint32_t j0 = nfull * step + tid * kVec;
+        const std::uint32_t j1 = j0 + kVec < n ? j0 + kVec : n;
+        if (j0 < n && kFactored) {  // the partial last step: this work-item's columns in it, one at a time
             for (int r = 0; r < RT; ++r) {
                 const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
                 float               part = 0.0f;
-                for (std::uint32_t j = j0; j < n; ++j) part = fmaf(St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], 1.0f), xv[j], part);
+                for (std::uint32_t j = j0; j < j1; ++j) part = fmaf(St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], 1.0f), xv[j], part);
                 add_comp(acc[r], comp[r], part);
             }
-        } else if (j0 < n) {  // the ragged tail of the rows (this work-item's last, partial chunk)
+        } else if (j0 < n) {  // the partial last step: this work-item's columns in it, one at a time
             for (int r = 0; r < RT; ++r) {
                 const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
                 float               part = 0.0f;
-                for (std::uint32_t j = j0; j < n; ++j) {
+                for (std::uint32_t j = j0; j < j1; ++j) {
                     const float av = St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], s[r]);
                     part           = fmaf(rank2(av, a1[r], v1p[j], a2[r], v2p[j]), xv[j], part);
                 }
@@ -890,9 +910,48 @@ public:
                 acc_ += part;
             }
         };
-        const std::uint32_t step = static_cast<std::uint32_t>(cs::wave::kSize) * kVec;
-        std::uint32_t       j0   = lane * kVec;
-        for (; j0 + kVec <= n; j0 += step) {
+        // the full column steps in this wave's rotated order; then the lane's columns of a partial last step
+        const std::uint32_t step  = static_cast<std::uint32_t>(cs::wave::kSize) * kVec;
+        std::uint32_t       j0    = lane * kVec;
+        const std::uint32_t nfull = n / step;
+        const std::uint32_t s0    = Rotation::of(r0 / static_cast<std::uint32_t>(W), nfull).first;
+        for (std::uint32_t t = 0; t < nfull && kStaged; ++t) {
+            // f32: v1 and the rows' chunks are issued first and fma(u1, v1, A) is taken row by row as the rows arrive;
+            // then v2 and x, then the rest of the rank-2 term and the dot (the same operations in the same order per
+            // element as the body below). This schedule measured 3% faster than the compiler's own, which waited for
+            // the rows before issuing v1, v2 and x one round trip at a time.
+            std::uint32_t sidx = s0 + t;
+            if (sidx >= nfull) sidx -= nfull;
+            j0 = sidx * step + lane * kVec;
+            float            v1[kVec], v2[kVec], x[kVec], b[W][kVec];
+            typename St::Raw raw[W];
+            load_f32<kVec>(v1p + j0, v1);
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                const std::uint32_t rr = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
+                raw[r]                 = load_stream<typename St::Raw>(A + static_cast<std::size_t>(r0 + rr) * lda + j0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < W; ++r) {
+                float av[kVec];
+                St::widen(raw[r], 1.0f, av);
+#pragma unroll
+                for (int v = 0; v < kVec; ++v) b[r][v] = fmaf(a1[r], v1[v], av[v]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            load_f32<kVec>(v2p + j0, v2);
+            load_f32<kVec>(xv + j0, x);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < W; ++r)
+#pragma unroll
+                for (int v = 0; v < kVec; ++v) acc[r] = fmaf(fmaf(a2[r], v2[v], b[r][v]), x[v], acc[r]);
+        }
+        for (std::uint32_t t = 0; t < nfull && !kStaged; ++t) {
+            std::uint32_t sidx = s0 + t;
+            if (sidx >= nfull) sidx -= nfull;
+            j0 = sidx * step + lane * kVec;
             float x[kVec], v1[kVec], v2[kVec];
             load_f32<kVec>(xv + j0, x);
             if constexpr (!kFactored) {
@@ -925,11 +984,13 @@ public:
                 }
             }
         }
-        if (j0 < n) {  // the ragged tail of the rows: this lane's last, partial chunk
+        j0 = nfull * step + lane * kVec;
+        if (j0 < n) {  // the partial last step: this lane's columns in it, one at a time
+            const std::uint32_t j1 = j0 + kVec < n ? j0 + kVec : n;
             for (int r = 0; r < W; ++r) {
                 const std::uint32_t rr   = static_cast<std::uint32_t>(r) < nr ? static_cast<std::uint32_t>(r) : 0u;
                 float               part = 0.0f;
-                for (std::uint32_t j = j0; j < n; ++j) {
+                for (std::uint32_t j = j0; j < j1; ++j) {
                     const float av = St::widen1(A[static_cast<std::size_t>(r0 + rr) * lda + j], kFactored ? 1.0f : s[r]);
                     part           = kFactored ? fmaf(av, xv[j], part) : fmaf(rank2(av, a1[r], v1p[j], a2[r], v2p[j]), xv[j], part);
                 }
