@@ -273,7 +273,8 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
             case kTop: {
                 const int p = q::phase_of(w.u, n);
                 std::uint32_t add[2];
-                if (q::to_publish(w.L, p, add)) {
+                const bool    published = q::to_publish(w.L, p, add);
+                if (published) {
                     st[q::done(0)] += add[0];
                     st[q::done(1)] += add[1];
                 }
@@ -283,7 +284,7 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
                         *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // a poll bound of 0 gives up at once and runs on
                     } else if (st[q::done(c)] < n.of(c)) {
                         ++g_queue_waits;
-                        return false;
+                        return published;  // a publish is progress (a wait it satisfies may be one tried before it)
                     }
                     w.L.seen |= 1u << c;
                 }
@@ -336,8 +337,12 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
             if (!moved) can.erase(can.begin() + static_cast<std::ptrdiff_t>(pick));
         }
         if (!moved) {
-            CHECK(false, "gemver: the work queue is stuck (G %u, resident %u, %u left, ticket %u, done %u/%u %u/%u)", G, resident, left,
-                  st[q::kTicket], st[q::done(0)], n.T1, st[q::done(1)], n.TR);
+            CHECK(false, "gemver: the work queue is stuck (G %u, resident %u, %u left, ticket %u, done %u/%u %u/%u, T2 %u)", G, resident, left,
+                  st[q::kTicket], st[q::done(0)], n.T1, st[q::done(1)], n.TR, n.T2);
+            for (std::uint32_t g = 0; g < G && g < 400; ++g)
+                if (wg[g].at != kNotStarted)
+                    std::printf("  wg %u at %d u %u next %u held %u %u seen %u fetches %u phase %d must_see %d\n", g, int(wg[g].at), wg[g].u, wg[g].next, wg[g].L.held[0],
+                                wg[g].L.held[1], wg[g].L.seen, wg[g].fetches, q::phase_of(wg[g].u, n), q::must_see(wg[g].L, q::phase_of(wg[g].u, n)));
             std::memset(st, 0, q::kWords * 4);
             return;
         }
