@@ -58,7 +58,7 @@ class Batch:
 def counts_of(sizes, R, RT):
     K = sizes.size
     n = sizes.astype(np.uint64)
-    ld = (n + 3) // 4 * 4
+    ld = (n + 31) // 32 * 32  # rows padded to 128-byte lines
     strips = (n + R - 1) // R
     tiles = (n + 4095) // 4096  # pass-1 tiles are R rows x 4096 columns
     blocks = (n + 1023) // 1024  # column-reduction blocks
