@@ -19,11 +19,12 @@
 // host-to-device upload and can be captured in a graph as it is.
 //
 // Ticket mode (a group with cs::group_waits_enabled or a cs::global_barrier): the state is a u32 array holding the
-// ticket counter, the finished counter and S segment-done counters, each on a 256-byte line of its own, then T per-group
-// done flags. Each work-group's leader takes the next ticket (so a group only ever waits on groups that have started: no
-// co-residency needed, reference kernel.hpp:197-205), waits are agent-scope acquire spins by the leader plus a barrier,
-// completion is a barrier, ONE release fence and the leader's atomics. The last group to finish zeroes the whole state,
-// so the next launch (or graph replay) starts from zero without a memset.
+// ticket counter, the finished counter, S segment-done counters and per-XCD write-back counters, each on a 256-byte line
+// of its own, then T per-group done flags. Each work-group's leader takes the next ticket (so a group only ever waits on
+// groups that have started: no co-residency needed, reference kernel.hpp:197-205), waits are agent-scope acquire spins by
+// the leader plus a barrier, completion is a barrier, the group's stores published by an L2 write-back that the
+// work-groups of one XCD share, and the leader's atomics. The last group to finish zeroes the whole state, so the next
+// launch (or graph replay) starts from zero without a memset.
 
 #include <hip/hip_runtime.h>
 
@@ -209,13 +210,18 @@ __device__ inline LogicalGroup locate_physical(std::uint32_t split, const Segmen
 
 // ---- ticket state ---------------------------------------------------------------------------------------------------
 // Word 0 the ticket counter, word kStateLine the finished counter, word (2 + s) kStateLine segment s's finished count,
-// then one done flag per ticket. Every work-group makes one atomic on each of the three counters it touches: on one
-// line they were served one after another by the same atomic unit (the reference's fused NS residual at 4096^2, 16384
-// groups: 1.94 ms with them packed, 0.85 ms with a line each).
-inline constexpr std::size_t kStateLine = 64;  // u32 words: 256 bytes
+// then per XCD x the started and completed counts of its shared L2 write-backs (words (2 + S + 2x) kStateLine and
+// (3 + S + 2x) kStateLine), then one done flag per ticket. Every work-group makes one atomic on each of the three
+// counters it touches: on one line they were served one after another by the same atomic unit (the reference's fused NS
+// residual at 4096^2, 16384 groups: 1.94 ms with them packed, 0.85 ms with a line each).
+inline constexpr std::size_t   kStateLine = 64;  // u32 words: 256 bytes
+inline constexpr std::uint32_t kXcds      = 8;   // XCDs of an MI355X (one L2 each)
 CS_HD inline std::size_t finished_word() { return kStateLine; }
 CS_HD inline std::size_t segment_word(std::size_t s) { return (2 + s) * kStateLine; }
-CS_HD inline std::size_t flag_word(std::size_t segments, std::uint64_t ticket) { return (2 + segments) * kStateLine + ticket; }
+CS_HD inline std::size_t write_back_word(std::size_t segments, std::uint32_t xcd, std::uint32_t completed) {
+    return (2 + segments + 2 * xcd + completed) * kStateLine;
+}
+CS_HD inline std::size_t flag_word(std::size_t segments, std::uint64_t ticket) { return (2 + segments + 2 * kXcds) * kStateLine + ticket; }
 CS_HD inline std::size_t ticket_words(std::size_t segments, std::uint64_t total) { return flag_word(segments, total); }
 
 /// A spin-wait load: relaxed at agent scope (coherent across the XCDs' L2s without invalidating them on every poll); the
@@ -253,6 +259,45 @@ __device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
+}
+
+/// The XCD this wave runs on (hardware register XCC_ID; work-group i of a launch lands on XCD i % 8).
+__device__ inline std::uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
+
+/// Publishes the work-group's stores to the other XCDs with a write-back of its XCD's L2 that it may share: a write-back
+/// of an L2 takes every dirty line in it, so one that STARTS after this group's stores have reached the L2 publishes them
+/// too, whichever work-group issues it. Per XCD, write-backs are numbered (`started`, then `completed` once done; one in
+/// flight at a time). The group reads s0 = started after its stores are in the L2 (every wave waited for them before
+/// the barrier), so write-back s0 + 1 starts after them; it returns once `completed` > s0, issuing that write-back itself
+/// when none is in flight. A release fence per work-group serialised them instead: the fused NS residual at 4096^2 with
+/// 16384 groups took 0.85 ms, 0.47 ms when only one group in eight wrote back (a timing probe, not a protocol). An XCD
+/// id past the table, or a poll that gives up (reported), falls back to the group's own write-back. Leader only.
+__device__ inline void publish_stores(std::uint32_t* ts, std::uint32_t segments, std::uint32_t* fault, std::uint32_t limit) {
+    const std::uint32_t x = xcd_id();
+    if (x >= kXcds) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        return;
+    }
+    std::uint32_t* const started   = &ts[write_back_word(segments, x, 0)];
+    std::uint32_t* const completed = &ts[write_back_word(segments, x, 1)];
+    const std::uint32_t  s0        = poll_load(started);
+    const bool           done      = bounded::poll(
+        [&] {
+            const std::uint32_t d = poll_load(completed);
+            if (d > s0) return true;
+            std::uint32_t s = poll_load(started);
+            if (s != d) return false;  // one in flight: it may have started before this group's stores arrived
+            if (!__hip_atomic_compare_exchange_strong(started, &s, s + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                return false;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the write-back, complete when the fence returns
+            __hip_atomic_store(completed, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;  // s == d <= s0 <= s: this is write-back s0 + 1
+        },
+        limit);
+    if (!done) {
+        bounded::report(fault, CS_HIP_FAULT_TICKET_WAIT);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    }
 }
 
 __device__ inline std::uint32_t local_linear_id() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
@@ -896,16 +941,18 @@ __global__ __launch_bounds__(MaxThreads) __attribute__((amdgpu_waves_per_eu(MinW
     run_stages<N>(ga, lg, cs_group_lds, std::make_index_sequence<N>{});
 
     if (ts != nullptr) {
-        // completion: barrier, ONE release fence (it orders the group's writes before the flag and the counts), the group's
-        // flag and its segment's count, then the finished count; only the last group acquires, and zeroes the state (an
-        // acquire per group would invalidate its XCD's L2 every time). The finished count must come after the flag and
-        // the count, or the last group could zero a word that one of them then sets: both are read-modify-writes whose
-        // results the empty asm consumes, so the finished count is issued only once both have been performed (a second
-        // release fence would order them too, at the cost of another L2 write-back per work-group).
+        // completion: every wave waits for its stores to reach the L2, barrier, the group's stores published (a shared
+        // write-back of the XCD's L2, publish_stores), the group's flag and its segment's count, then the finished count;
+        // only the last group acquires, and zeroes the state (an acquire per group would invalidate its XCD's L2 every
+        // time). The finished count must come after the flag and the count, or the last group could zero a word that one
+        // of them then sets: both are read-modify-writes whose results the empty asm consumes, so the finished count is
+        // issued only once both have been performed (a release fence would order them too, at the cost of another L2
+        // write-back per work-group).
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         __shared__ std::uint32_t s_last;
         if (local_linear_id() == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            publish_stores(ts, geo.segments, geo.fault, geo.spin_limit);
             const std::uint32_t was = __hip_atomic_exchange(&ts[flag_word(geo.segments, ticket)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const std::uint32_t cnt = __hip_atomic_fetch_add(&ts[segment_word(lg.segment)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             asm volatile("" ::"v"(was), "v"(cnt) : "memory");

@@ -127,6 +127,30 @@ def test_repeated_runs_reset_the_accumulators(variant):
     assert st[3] >= 1, st  # replayed
 
 
+def test_fused_waits_publish_each_runs_partials():
+    """The fused variant with group waits reads every tile's partials in later segments of the same launch, after the
+    tiles published them (an L2 write-back shared by the work-groups of an XCD). Four different states in turn through
+    one model (eager, captured, then replayed), 2048 tiles over every XCD: the partials and diagnostics equal the
+    launch-boundary variant's on the same state bit for bit, so none was read stale from the run before."""
+    rng = np.random.default_rng(5)
+    nx, ny = 1024, 1024
+    base = O.nsc_init(nx, ny)
+    fused, ref = Nsc("fused", nx, ny), Nsc("launches", nx, ny)
+    try:
+        for it in range(4):
+            U = (base * (1 + 0.05 * (it + 1) * rng.standard_normal(base.shape))).astype(np.float32)
+            for m in (fused, ref):
+                m.write(U)
+                m.run(2 if it == 3 else 1)
+            print("state", it, "stats", fused.stats(), flush=True)
+            got, want = fused.outputs(), ref.outputs()
+            for k in ("R", "Part", "DetDt", "DetWs", "DetSq", "Dt", "Ws"):  # the atomic sums' order is free: Sq, Err
+                np.testing.assert_array_equal(got[k], want[k], err_msg=f"state {it} {k}")
+    finally:
+        fused.close()
+        ref.close()
+
+
 def test_host_state_with_random_perturbation():
     rng = np.random.default_rng(11)
     U = O.nsc_init(256, 64)
