@@ -271,15 +271,18 @@ __device__ inline std::uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((20)
 /// the barrier), so write-back s0 + 1 starts after them; it returns once `completed` > s0, issuing that write-back itself
 /// when none is in flight. A release fence per work-group serialised them instead: the fused NS residual at 4096^2 with
 /// 16384 groups took 0.85 ms, 0.47 ms when only one group in eight wrote back (a timing probe, not a protocol). An XCD
-/// id past the table, or a poll that gives up (reported), falls back to the group's own write-back. Leader only.
-__device__ inline void publish_stores(std::uint32_t* ts, std::uint32_t segments, std::uint32_t* fault, std::uint32_t limit) {
+/// id past the table, or a poll that gives up, falls back to the group's own write-back (false: the caller reports).
+/// `wb`: the 2 kXcds counters (XCD x's started at wb[2 x kStateLine], completed at wb[(2 x + 1) kStateLine]), zero at
+/// the launch. Every counter update of the call has been performed when it returns, so the caller's later atomics (its
+/// finished count, after which the last work-group zeroes the counters) come after them. Leader only.
+__device__ inline bool publish_stores(std::uint32_t* wb, std::uint32_t limit) {
     const std::uint32_t x = xcd_id();
     if (x >= kXcds) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        return;
+        return true;
     }
-    std::uint32_t* const started   = &ts[write_back_word(segments, x, 0)];
-    std::uint32_t* const completed = &ts[write_back_word(segments, x, 1)];
+    std::uint32_t* const started   = &wb[2 * x * kStateLine];
+    std::uint32_t* const completed = &wb[(2 * x + 1) * kStateLine];
     const std::uint32_t  s0        = poll_load(started);
     const bool           done      = bounded::poll(
         [&] {
@@ -289,15 +292,14 @@ __device__ inline void publish_stores(std::uint32_t* ts, std::uint32_t segments,
             if (s != d) return false;  // one in flight: it may have started before this group's stores arrived
             if (!__hip_atomic_compare_exchange_strong(started, &s, s + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 return false;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the write-back, complete when the fence returns
-            __hip_atomic_store(completed, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the write-back, ordered before the count that follows
+            __hip_atomic_exchange(completed, s + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return true;  // s == d <= s0 <= s: this is write-back s0 + 1
         },
         limit);
-    if (!done) {
-        bounded::report(fault, CS_HIP_FAULT_TICKET_WAIT);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    }
+    if (!done) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the counter updates above are performed
+    return done;
 }
 
 __device__ inline std::uint32_t local_linear_id() { return threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z); }
@@ -952,7 +954,7 @@ __global__ __launch_bounds__(MaxThreads) __attribute__((amdgpu_waves_per_eu(MinW
         __syncthreads();
         __shared__ std::uint32_t s_last;
         if (local_linear_id() == 0) {
-            publish_stores(ts, geo.segments, geo.fault, geo.spin_limit);
+            if (!publish_stores(&ts[write_back_word(geo.segments, 0, 0)], geo.spin_limit)) bounded::report(geo.fault, CS_HIP_FAULT_TICKET_WAIT);
             const std::uint32_t was = __hip_atomic_exchange(&ts[flag_word(geo.segments, ticket)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             const std::uint32_t cnt = __hip_atomic_fetch_add(&ts[segment_word(lg.segment)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             asm volatile("" ::"v"(was), "v"(cnt) : "memory");
