@@ -29,14 +29,14 @@
 //   RaggedPass1 / RaggedReduce / RaggedPass2  one plain launch per phase, work-group u runs that phase's unit u: no
 //                  work queue and no waits between work-groups; the launch boundaries order the phases.
 //   persistent
-//   RaggedGemver   ONE launch: a grid of G work-groups takes units by ticket until none is left, first every pass-1 tile of
-//                  every instance, then every column reduction, then every pass-2 row group (the queue protocol below).
-//                  A work-group publishes the units it ran when it first takes a unit of a later phase (one release, one
-//                  add per phase counter), and waits once per phase for the phase before it to be complete (one acquire):
-//                  at most two releases and two acquires per work-group, none per unit. Waits are only ever on units with
-//                  earlier tickets, held by work-groups that are running and publish them before they wait themselves, so
-//                  the queue drains without co-residency. The last work-group to leave zeroes the counters (graph
-//                  replays need no memset).
+//   RaggedGemver   ONE launch: a grid of G work-groups takes units by ticket until none is left, first every pass-1 unit of
+//                  every instance, then every column reduction, then every pass-2 row group, each in the work order (the
+//                  queue protocol below). A unit waits for its own instance's units of the phase before (a per-instance
+//                  ready count), so the phases overlap; a pass-1 unit or reduction publishes its stores with an L2
+//                  write-back shared by its XCD's work-groups and counts itself, and the units that wait read P and x with
+//                  agent-scope loads: no acquire. Waits are only ever on units with earlier tickets, held by work-groups
+//                  that are running and never wait on later ones, so the queue drains without co-residency. The last
+//                  work-group to leave zeroes the counters (graph replays need no memset).
 // The arithmetic of a unit is the two-pass form of gemver_components.hpp (AhatTPartial / ColReduce / AhatGemv): x_k equals
 // the reference components' f32 x for the same R bit for bit, in both forms. R, RT (4, 8: the work-group shares a row
 // tile; 16, 32: each wave owns RT / 4 rows) and G are run-time parameters (the autotuner's knobs; G only for the
@@ -86,10 +86,11 @@ inline void count_units(const std::uint32_t* sizes, std::size_t K, std::uint32_t
     }
 }
 
-/// What a unit needs to start: its instance's size and offsets (vector elements, 16-byte units of A, P elements) and its
-/// index among the instance's units of its phase.
+/// What a unit needs to start: its instance's size and offsets (vector elements, 16-byte units of A, P elements), its
+/// index among the instance's units of its phase, and the instance (set from the unit map).
 struct Unit {
     std::uint32_t n, t, vo, a4, poff;
+    std::uint32_t k = 0;
 };
 
 /// The device scan's offset columns (column c of instance k = scan[cK + k] - scan[cK]).
@@ -353,27 +354,37 @@ __device__ inline bool map_unit(const std::uint32_t* map, std::size_t map_elems,
     using U4    = std::uint32_t __attribute__((ext_vector_type(4)));
     const U4 e0 = reinterpret_cast<const U4*>(map)[2 * static_cast<std::size_t>(u)];
     const U4 e1 = reinterpret_cast<const U4*>(map)[2 * static_cast<std::size_t>(u) + 1];
-    w           = Unit{e0[0], e0[1], e0[2], e0[3], e1[0]};
+    w           = Unit{e0[0], e0[1], e0[2], e0[3], e1[0], e1[1]};
     return u < total && static_cast<std::size_t>(total) * kMapWords <= map_elems;
 }
 
 /// The persistent form's work queue, as host-and-device functions (the kernel runs them, and so does its CPU emulation,
-/// which interleaves simulated work-groups adversarially). Tickets run through three phases: [0, T1) pass-1 tiles,
-/// [T1, T1 + TR) column reductions, [T1 + TR, T1 + TR + T2) pass-2 row groups; any later ticket means "leave". A
-/// reduction of instance k needs all of k's pass-1 tiles and a pass-2 group all of k's reductions; the queue orders
-/// whole phases instead (a work-group's first unit of phase p waits until phase p - 1 is complete), which needs no
-/// per-unit release: state word done(q) counts the phase-q units published so far. The four counters sit 256 bytes apart,
-/// so the work-groups polling a done counter at a phase's end do not contend with the ticket fetches of those still working.
+/// which interleaves simulated work-groups adversarially). Tickets run through three phases: [0, T1) pass-1 units,
+/// [T1, T1 + TR) column reductions, [T1 + TR, T1 + TR + T2) pass-2 row groups, each phase in the work order (largest
+/// instances first); any later ticket means "leave". A unit waits for its own instance only: a reduction of instance k
+/// until all of k's pass-1 units are published, a pass-2 group until all of k's reductions are (state word ready(p, k)
+/// counts instance k's published phase-p units), so the phases overlap: the reductions and pass-2 groups of the largest
+/// instances run while the last pass-1 units of the smallest still do. A unit of phase 0 or 1 publishes its stores with a
+/// write-back of its XCD's L2 shared with the XCD's other work-groups, then counts itself in its instance's ready word
+/// and in its phase's published word. A wait ends with one acquire by the leader; a work-group that has seen a whole phase
+/// published (published(q) = its unit count) acquires once more and then neither polls nor acquires for it again. Every
+/// wait targets units with earlier tickets, taken by running work-groups that finish them without waiting on later ones:
+/// no co-residency needed. The counters sit 256 bytes apart (ticket, finished, the two phases' published counts, the
+/// write-back counts), the K + K ready counts after them.
 namespace queue {
-inline constexpr int kStride = 64;  ///< u32 words between two counters
-inline constexpr int kTicket = 0, kFinished = kStride, kWords = 4 * kStride;
-CS_HD constexpr int done(int q) { return (2 + q) * kStride; }
+inline constexpr int kStride    = 64;  ///< u32 words between two counters
+inline constexpr int kTicket    = 0, kFinished = kStride;
+CS_HD constexpr int published(int q) { return (2 + q) * kStride; }
+inline constexpr int kWriteBack = 4 * kStride;                                                ///< 2 counters per XCD
+inline constexpr int kReady     = kWriteBack + 2 * static_cast<int>(cs::detail::kXcds) * kStride;  ///< then ready(p, k)
+CS_HD constexpr std::size_t words(std::size_t K) { return kReady + 2 * K; }
+CS_HD constexpr std::size_t ready(int p, std::size_t k, std::size_t K) { return kReady + static_cast<std::size_t>(p) * K + k; }
 
 struct Totals {
     std::uint32_t T1 = 0, TR = 0, T2 = 0;
     CS_HD std::uint32_t all() const { return T1 + TR + T2; }
-    /// units of phase q (0: pass 1, 1: reductions)
-    CS_HD std::uint32_t of(int q) const { return q == 0 ? T1 : TR; }
+    /// units of phase p
+    CS_HD std::uint32_t of(int p) const { return p == 0 ? T1 : p == 1 ? TR : T2; }
     /// the first ticket of phase p
     CS_HD std::uint32_t base(int p) const { return p == 0 ? 0u : p == 1 ? T1 : T1 + TR; }
 };
@@ -387,33 +398,10 @@ CS_HD inline int phase_of(std::uint32_t t, const Totals& n) {
     return t < n.T2 ? 2 : 3;
 }
 
-/// What a work-group's leader keeps between units: the units of phases 0 and 1 it ran and has not published yet, and the
-/// phases it has seen complete (bit q).
-struct Leader {
-    std::uint32_t held[2] = {0, 0};
-    std::uint32_t seen    = 0;
-};
-
-/// Before a unit of phase p (p = 3: before leaving), the counts the leader publishes now: every held unit of a phase
-/// before p. True when there is any (the leader then releases once and adds add[q] to state word done(q)).
-CS_HD inline bool to_publish(Leader& L, int p, std::uint32_t (&add)[2]) {
-    add[0] = p >= 1 ? L.held[0] : 0u;
-    add[1] = p >= 2 ? L.held[1] : 0u;
-    L.held[0] -= add[0];
-    L.held[1] -= add[1];
-    return (add[0] | add[1]) != 0u;
-}
-
-/// The phase that must be complete before a unit of phase p runs, or -1: nothing before a pass-1 tile or before leaving,
-/// and a phase already seen complete once is not waited for again.
-CS_HD inline int must_see(const Leader& L, int p) {
-    const int q = p == 1 ? 0 : p == 2 ? 1 : -1;
-    return q >= 0 && ((L.seen >> q) & 1u) == 0u ? q : -1;
-}
-
-/// After running a unit of phase p.
-CS_HD inline void ran(Leader& L, int p) {
-    if (p < 2) ++L.held[p];
+/// How many phase-(p - 1) units of an instance of size n a phase-p unit of it waits for: its pass-1 units (p = 1), its
+/// column reductions (p = 2); none for pass 1.
+CS_HD inline std::uint32_t awaits(int p, std::uint32_t n, std::uint32_t R) {
+    return p == 1 ? ceil_div(n, R) * ceil_div(n, kTile1) : p == 2 ? ceil_div(n, kCols) : 0u;
 }
 }  // namespace queue
 
@@ -508,46 +496,48 @@ public:
     };
     struct Outputs {
         cs::Output<DType::f32, 1> P, x, w;
-        cs::Output<DType::u32, 1> state;  ///< ticket, finished, pass-1 tiles published, reductions published (queue::)
+        cs::Output<DType::u32, 1> state;  ///< ticket, finished, write-back counts, per-instance ready counts (queue::)
     };
 
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         const Arrays        a = arrays(in, out);
-        const Cols          cols{cs::raw::data(in.offs), static_cast<std::uint32_t>(in.sizes.size())};
+        const std::uint32_t K = static_cast<std::uint32_t>(in.sizes.size());
+        const Cols          cols{cs::raw::data(in.offs), K};
         const queue::Totals n{cols.total(3), cols.total(4), cols.total(5)};
         const std::uint32_t T   = n.all();
         const std::uint32_t RT  = in.params[std::size_t{1}];
         std::uint32_t*      st  = cs::raw::data(out.state);
         const std::uint32_t tid = static_cast<std::uint32_t>(it.get_local_id(0));
+        const std::uint32_t* const maps[3]  = {cs::raw::data(in.map1), cs::raw::data(in.mapR), cs::raw::data(in.map2)};
+        const std::size_t          sizes[3] = {in.map1.size(), in.mapR.size(), in.map2.size()};
         __shared__ std::uint32_t s_unit;
 
-        // Leader-only blocks sit between barriers: the first fetch; per iteration the publish / wait / next fetch before
-        // the unit and the hand-over of the fetched ticket after it (so no two of them can be merged across the loop's
-        // back edge), and the exit is decided from LDS after a barrier (uniform). The next ticket is fetched after the
-        // wait and before the unit, so its latency hides under the unit, and a work-group never waits while it holds an
-        // unstarted ticket of an earlier phase. A work-group takes each unit at most once: T + 1 fetches bound the loop.
-        queue::Leader L;       // meaningful in the leader only
-        std::uint32_t next = 0;
+        // Leader-only blocks sit between barriers: the first fetch; per iteration the wait and the next fetch before the
+        // unit, the publication and the hand-over of the fetched ticket after it (so no two of them can be merged across
+        // the loop's back edge), and the exit is decided from LDS after a barrier (uniform). The next ticket is fetched
+        // after the wait and before the unit, so its latency hides under the unit. A work-group takes each unit at most
+        // once: T + 1 fetches bound the loop.
+        std::uint32_t next = 0, seen = 0;  // leader only: seen bit q = phase q seen wholly published (and acquired after)
         if (tid == 0) s_unit = __hip_atomic_fetch_add(&st[queue::kTicket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         std::uint32_t u       = detail::uniform(s_unit);
         bool          drained = false;
         for (std::uint32_t fetches = 1; fetches <= T + 1; ++fetches) {
             const int p = queue::phase_of(u, n);
+            Unit      w{0, 0, 0, 0, 0};
+            const bool has = p < 3 && map_unit(maps[p], sizes[p], u - n.base(p), n.of(p), w);
             if (tid == 0) {
-                std::uint32_t add[2];
-                if (queue::to_publish(L, p, add)) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    for (int q = 0; q < 2; ++q)
-                        if (add[q] != 0u) __hip_atomic_fetch_add(&st[queue::done(q)], add[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                const int q = queue::must_see(L, p);
-                if (q >= 0) {
-                    const std::uint32_t need = n.of(q);
-                    if (!cs::bounded::poll([&] { return cs::detail::poll_load(&st[queue::done(q)]) >= need; }, it.spin_limit()))
-                        it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
+                const int q = p - 1;
+                if (has && p > 0 && ((seen >> q) & 1u) == 0u) {  // the instance's units of the phase before, all published
+                    if (cs::detail::poll_load(&st[queue::published(q)]) >= n.of(q)) {
+                        seen |= 1u << q;  // the whole phase: no poll and no acquire for it from now on
+                    } else {
+                        const std::uint32_t  need = queue::awaits(p, w.n, a.R);
+                        std::uint32_t* const c    = &st[queue::ready(q, w.k, K)];
+                        if (!cs::bounded::poll([&] { return cs::detail::poll_load(c) >= need; }, it.spin_limit()))
+                            it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
+                    }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                    L.seen |= 1u << q;
                 }
                 if (p != 3) next = __hip_atomic_fetch_add(&st[queue::kTicket], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
@@ -555,47 +545,48 @@ public:
                 drained = true;
                 break;
             }
-            __syncthreads();  // the leader's wait (and acquire) is behind every work-item
-            run_unit(a, in, p, u - n.base(p), n, RT, tid);
+            __syncthreads();  // the leader's wait is behind every work-item
+            if (has) run_unit(a, p, w, RT, tid);
+            if (p < 2) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this wave's stores have reached the L2
+            __syncthreads();
             if (tid == 0) {
-                queue::ran(L, p);
+                if (p < 2) {
+                    if (!cs::detail::publish_stores(&st[queue::kWriteBack], it.spin_limit())) it.report_fault(CS_HIP_FAULT_UNIT_WAIT);
+                    if (has) {
+                        __hip_atomic_fetch_add(&st[queue::ready(p, w.k, K)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_fetch_add(&st[queue::published(p)], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
                 s_unit = next;
             }
             __syncthreads();
             u = detail::uniform(s_unit);
         }
-        // leaving: every unit this group ran is published (the p = 3 step above); the last work-group to leave zeroes the
-        // state (every unit is done and nobody reads it any more). A loop ended by the fetch bound found a state that was
-        // not zero at the launch: it publishes what it holds (so nobody waits for it) and reports.
+        // leaving: the last work-group to leave zeroes the state (every unit is done and nobody reads it any more); its
+        // count waits for this group's own counter updates. A loop ended by the fetch bound found a state that was not
+        // zero at the launch, and reports.
         __syncthreads();
         __shared__ std::uint32_t s_last;
         if (tid == 0) {
-            if (!drained) {
-                std::uint32_t add[2];
-                if (queue::to_publish(L, 3, add)) {
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                    for (int q = 0; q < 2; ++q)
-                        if (add[q] != 0u) __hip_atomic_fetch_add(&st[queue::done(q)], add[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                it.report_fault(CS_HIP_FAULT_STATE);
-            }
-            const std::uint32_t f = __hip_atomic_fetch_add(&st[queue::kFinished], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (!drained) it.report_fault(CS_HIP_FAULT_STATE);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            const std::uint32_t f = __hip_atomic_fetch_add(&st[queue::kFinished], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_last                = f + 1 == static_cast<std::uint32_t>(it.get_group_range(0)) ? 1u : 0u;
         }
         __syncthreads();
-        if (s_last && tid < 4) __hip_atomic_store(&st[tid * queue::kStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_last) {
+            const std::size_t words = queue::words(K);
+            for (std::size_t i = tid; i < words; i += kWg) __hip_atomic_store(&st[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 
-    /// Unit i of phase p, described by its unit-map entry (a map that cannot hold the phase's count runs nothing: the
-    /// host sizes the maps for every unit count it allows).
-    __device__ static void run_unit(const Arrays& a, const Inputs& in, int p, std::uint32_t i, const queue::Totals& n, std::uint32_t RT,
-                                    std::uint32_t tid) {
-        Unit w;
+    /// A unit of phase p.
+    __device__ static void run_unit(const Arrays& a, int p, const Unit& w, std::uint32_t RT, std::uint32_t tid) {
         if (p == 0) {
-            if (map_unit(cs::raw::data(in.map1), in.map1.size(), i, n.T1, w)) unit_pass1(a, w, tid);
+            unit_pass1(a, w, tid);
         } else if (p == 1) {
-            if (map_unit(cs::raw::data(in.mapR), in.mapR.size(), i, n.TR, w)) unit_reduce(a, w, tid);
-        } else if (map_unit(cs::raw::data(in.map2), in.map2.size(), i, n.T2, w)) {
+            unit_reduce(a, w, tid);
+        } else {
             switch (RT) {
                 case 4: unit_pass2<4>(a, w, tid); break;
                 case 8: unit_pass2<8>(a, w, tid); break;
@@ -643,11 +634,12 @@ public:
         declare_output(outputs.P, "P", cs::unspecified_shape<1>());
         declare_output(outputs.x, "x", cs::unspecified_shape<1>());
         declare_output(outputs.w, "w", cs::unspecified_shape<1>());
-        declare_scratch(outputs.state, "state", queue::kWords);
+        declare_scratch(outputs.state, "state", cs::unspecified_shape<1>());
     }
     void prepare_forward() override {
         set_shape(outputs.x, shape_of(inputs.u1)[0]);
         set_shape(outputs.w, shape_of(inputs.u1)[0]);
+        set_shape(outputs.state, queue::words(shape_of(inputs.sizes)[0]));
     }
     /// G persistent work-groups (the grid, the autotuner's third knob); every phase's unit count must fit its map.
     std::unique_ptr<cs::LaunchDescription> define_forward() override {

@@ -194,10 +194,11 @@ static void check_map(const Emu& e, const std::uint32_t* map, std::size_t map_el
 // The kernel's queue decisions are batch::queue's host-and-device functions; here G simulated work-groups step through
 // them (first fetch; per unit: publish, wait, fetch the next ticket, run, hand over; leave) in an order a seeded generator
 // picks, with at most `resident` work-groups started at a time (the rest start only when one leaves: no co-residency).
-// A unit runs at once when its step is taken, so the checks are that every reduction of an instance comes after all of
-// its pass-1 tiles and every pass-2 group after all of its reductions, that a work-group never fetches more than T + 1
-// tickets, that the queue always drains (some work-group can step until all have left), and that it leaves the state
-// zeroed.
+// A unit runs at once when its step is taken and publishes (counts itself in its instance's ready word) when it ends,
+// so the checks are that every reduction of an instance comes after all of its pass-1 units and every pass-2 group
+// after all of its reductions, that the unit map names each unit's instance, that a work-group never fetches more than
+// T + 1 tickets, that the queue always drains (some work-group can step until all have left), and that it leaves the
+// state zeroed.
 static std::uint64_t g_rng = 0x9e3779b97f4a7c15ull;
 static std::uint32_t next_rand() {
     g_rng ^= g_rng << 13;
@@ -215,12 +216,13 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
     CHECK(grid[0] == G && grid[1] == 1 && grid[2] == 1 && block[0] == batch::kWg, "gemver: grid %u %u %u block %u for G %u", grid[0],
           grid[1], grid[2], block[0], G);
     const std::uint32_t K = static_cast<std::uint32_t>(a.in.sizes.size());
-    CHECK(a.out.state.size() == q::kWords, "gemver: state holds %zu words, not %d", a.out.state.size(), q::kWords);
+    const std::size_t words = q::words(K);
+    CHECK(a.out.state.size() == words, "gemver: state holds %zu words, not %zu", a.out.state.size(), words);
     std::uint32_t* st   = cs::raw::data(a.out.state);
     bool           zero = true;
-    for (int w = 0; w < q::kWords; ++w) zero = zero && st[w] == 0;
-    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u, done %u %u)",
-          (unsigned long long)g_gemver_launches, st[q::kTicket], st[q::kFinished], st[q::done(0)], st[q::done(1)]);
+    for (std::size_t w = 0; w < words; ++w) zero = zero && st[w] == 0;
+    CHECK(zero, "gemver launch %llu: the work-queue state is not zero at the start (ticket %u, finished %u)",
+          (unsigned long long)g_gemver_launches, st[q::kTicket], st[q::kFinished]);
     if (!zero) return;
 
     // the launch carries the device's fault word and the current poll bound; a bound of 0 makes every wait give up
@@ -249,8 +251,17 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
     enum Where { kNotStarted, kTop, kRun, kLeft };
     struct WG {
         Where         at      = kNotStarted;
-        q::Leader     L;
-        std::uint32_t u = 0, next = 0, fetches = 0;
+        std::uint32_t u = 0, next = 0, fetches = 0, seen = 0;
+    };
+    const std::uint32_t R = e.R;
+    // the instance of unit i of phase p, from the unit map (checked against the scan)
+    auto instance = [&](int p, std::uint32_t i) {
+        check_map(e, maps[p], map_el[p], p, i, "gemver");
+        std::uint32_t       t;
+        const std::uint32_t k = e.locate(p, i, t);
+        CHECK(maps[p][batch::kMapWords * std::size_t{i} + 5] == k, "gemver: unit %u of phase %d names instance %u, the scan %u", i, p,
+              maps[p][batch::kMapWords * std::size_t{i} + 5], k);
+        return k;
     };
     std::vector<WG>            wg(G);
     std::vector<std::uint32_t> p1(K, 0), red(K, 0);
@@ -272,27 +283,23 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
                 return true;
             case kTop: {
                 const int p = q::phase_of(w.u, n);
-                std::uint32_t add[2];
-                const bool    published = q::to_publish(w.L, p, add);
-                if (published) {
-                    st[q::done(0)] += add[0];
-                    st[q::done(1)] += add[1];
-                }
-                const int c = q::must_see(w.L, p);
-                if (c >= 0) {
-                    if (give_up) {
-                        *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // a poll bound of 0 gives up at once and runs on
-                    } else if (st[q::done(c)] < n.of(c)) {
-                        ++g_queue_waits;
-                        return published;  // a publish is progress (a wait it satisfies may be one tried before it)
-                    }
-                    w.L.seen |= 1u << c;
-                }
                 if (p == 3) {
                     w.at = kLeft;
-                    if (++st[q::kFinished] == G) std::memset(st, 0, q::kWords * 4);  // the last to leave zeroes the state
+                    if (++st[q::kFinished] == G) std::memset(st, 0, words * 4);  // the last to leave zeroes the state
                     ++left;
                     return true;
+                }
+                const int c = p - 1;
+                if (p > 0 && ((w.seen >> c) & 1u) == 0u) {  // the instance's units of the phase before, all published
+                    const std::uint32_t i = w.u - n.base(p), k = instance(p, i);
+                    if (st[q::published(c)] >= n.of(c)) {
+                        w.seen |= 1u << c;
+                    } else if (give_up) {
+                        *geo.fault |= CS_HIP_FAULT_UNIT_WAIT;  // a poll bound of 0 gives up at once and runs on
+                    } else if (st[q::ready(c, k, K)] < q::awaits(p, e.sz[k], R)) {
+                        ++g_queue_waits;
+                        return false;
+                    }
                 }
                 w.next = fetch(w);
                 w.at   = kRun;
@@ -300,23 +307,22 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
             }
             case kRun: {
                 const int           p = q::phase_of(w.u, n);
-                const std::uint32_t i = w.u - n.base(p);
-                check_map(e, maps[p], map_el[p], p, i, "gemver");
+                const std::uint32_t i = w.u - n.base(p), k = instance(p, i);
                 if (p == 0) {
-                    ++p1[e.pass1(i)];
+                    CHECK(e.pass1(i) == k, "gemver: pass-1 unit %u ran for another instance", i);
+                    ++p1[k];
                 } else if (p == 1) {
-                    std::uint32_t       t;
-                    const std::uint32_t k = e.locate(1, i, t);
-                    CHECK(give_up || p1[k] == e.s[3 * K + k + 1] - e.s[3 * K + k], "gemver: reduction of instance %u before its pass-1 tiles", k);
+                    CHECK(give_up || p1[k] == e.s[3 * K + k + 1] - e.s[3 * K + k], "gemver: reduction of instance %u before its pass-1 units", k);
                     ++red[e.reduce(i)];
                 } else {
-                    std::uint32_t       t;
-                    const std::uint32_t k = e.locate(2, i, t);
                     CHECK(give_up || red[k] == e.s[4 * K + k + 1] - e.s[4 * K + k], "gemver: pass 2 of instance %u before its reductions", k);
                     e.pass2(i);
                 }
+                if (p < 2) {  // published
+                    ++st[q::ready(p, k, K)];
+                    ++st[q::published(p)];
+                }
                 ++g_queue_runs;
-                q::ran(w.L, p);
                 w.u  = w.next;
                 w.at = kTop;
                 return true;
@@ -337,17 +343,16 @@ static void emulate_gemver(const void* bytes, const std::uint32_t grid[3], const
             if (!moved) can.erase(can.begin() + static_cast<std::ptrdiff_t>(pick));
         }
         if (!moved) {
-            CHECK(false, "gemver: the work queue is stuck (G %u, resident %u, %u left, ticket %u, done %u/%u %u/%u, T2 %u)", G, resident, left,
-                  st[q::kTicket], st[q::done(0)], n.T1, st[q::done(1)], n.TR, n.T2);
+            CHECK(false, "gemver: the work queue is stuck (G %u, resident %u, %u left, ticket %u, T %u %u %u)", G, resident, left, st[q::kTicket], n.T1,
+                  n.TR, n.T2);
             for (std::uint32_t g = 0; g < G && g < 400; ++g)
                 if (wg[g].at != kNotStarted)
-                    std::printf("  wg %u at %d u %u next %u held %u %u seen %u fetches %u phase %d must_see %d\n", g, int(wg[g].at), wg[g].u, wg[g].next, wg[g].L.held[0],
-                                wg[g].L.held[1], wg[g].L.seen, wg[g].fetches, q::phase_of(wg[g].u, n), q::must_see(wg[g].L, q::phase_of(wg[g].u, n)));
-            std::memset(st, 0, q::kWords * 4);
+                    std::printf("  wg %u at %d u %u next %u fetches %u phase %d\n", g, int(wg[g].at), wg[g].u, wg[g].next, wg[g].fetches, q::phase_of(wg[g].u, n));
+            std::memset(st, 0, words * 4);
             return;
         }
     }
-    for (int w = 0; w < q::kWords; ++w) CHECK(st[w] == 0, "gemver: state word %d is %u after the launch", w, st[w]);
+    for (std::size_t w = 0; w < words; ++w) CHECK(st[w] == 0, "gemver: state word %zu is %u after the launch", w, st[w]);
 }
 
 // ---- the phased form: one launch per phase, work-group u runs unit u; the grid must cover the device plan's count ----
