@@ -295,7 +295,7 @@ int cs_batch_autotune(cs_batch g, int reps, const char* cache_path, uint32_t* be
             cs::detail::hip_check(cs_hip_device_info(0, nullptr, 0, &cus, nullptr, nullptr), "device info");
             float         best = 1e30f;
             std::uint32_t bp[3] = {64, 8, static_cast<std::uint32_t>(4 * cus)};
-            const std::vector<std::uint32_t> gms = g->persistent ? std::vector<std::uint32_t>{2u, 4u, 8u} : std::vector<std::uint32_t>{4u};
+            const std::vector<std::uint32_t> gms = g->persistent ? std::vector<std::uint32_t>{2u, 3u, 4u} : std::vector<std::uint32_t>{4u};
             const std::vector<std::uint32_t> Rs{64u, 128u, 256u};
             const std::vector<std::uint32_t> RTs{8u, 16u, 32u};
             for (std::uint32_t R : Rs)

@@ -187,7 +187,7 @@ int cs_batch_set_params(cs_batch b, uint32_t R, uint32_t RT, uint32_t groups);
 int cs_batch_get_params(cs_batch b, uint32_t* params3);
 /* 0: the phased form (default), 1: the persistent work-queue form; a change rebuilds the model at its next use */
 int cs_batch_set_mode(cs_batch b, int persistent);
-/* times every candidate (R in {64, 128, 256}, RT in {8, 16, 32}; persistent form also G in {2, 4, 8} x CUs) on this
+/* times every candidate (R in {64, 128, 256}, RT in {8, 16, 32}; persistent form also G in {2, 3, 4} x CUs) on this
  * model and keeps the fastest; cache_path (or NULL): a text file of "key R RT G" lines keyed by device, sizes and form,
  * read first and appended to */
 int cs_batch_autotune(cs_batch b, int reps, const char* cache_path, uint32_t* best3);
