@@ -444,11 +444,10 @@ public:
     static constexpr int kChunk = kWg * kEl;
     /// columns of one work-group
     static constexpr int kCols = kChunk * kV;
-    /// e4m3 storage sums the stored values against s_i y_i and adds the rank-2 terms once per column:
-    /// P[c][j] = sum_i q_ij (s_i y_i) + v1_j sum_i u1_i y_i + v2_j sum_i u2_i y_i (one fma per element instead of three and
-    /// a scale). f32 keeps the components' order (bit-exact); bf16 keeps it too (its passes are bound by HBM either way,
-    /// and the factored form measured slower for it).
-    static constexpr bool kFactored = S == DType::e4m3;
+    /// Reduced-precision storage sums the stored values against s_i y_i (s_i = 1 for bf16) and adds the rank-2 terms once
+    /// per column: P[c][j] = sum_i q_ij (s_i y_i) + v1_j sum_i u1_i y_i + v2_j sum_i u2_i y_i (one fma per element
+    /// instead of three and a scale). f32 keeps the components' order (bit-exact).
+    static constexpr bool kFactored = S != DType::f32;
     /// rows in flight per work-item: 16 loads of 16 bytes (f32 4 rows x 4 chunks, bf16 8 x 2, e4m3 16 x 1)
     static constexpr int kRowsInFlight = 16 / kV;
 
@@ -688,11 +687,13 @@ public:
     static constexpr int kVec                = St::kVec;
     static constexpr int kRows               = RT;
     static constexpr bool kCompensated       = S != DType::f32;
-    /// e4m3 storage: w_i = alpha (s_i sum_j q_ij x_j + u1_i (v1 . x) + u2_i (v2 . x)) with the two dots from the column
-    /// reduction (one fma per stored element instead of three and a scale, and no v1, v2 re-reads per work-group: at one
-    /// byte per element those re-reads outweighed A)
-    static constexpr bool kFactored = S == DType::e4m3;
-    /// column chunks in flight per work-item for e4m3: 16 loads of 16 bytes of A (RT rows each); f32 and bf16 as tuned
+    /// Reduced-precision storage: w_i = alpha (s_i sum_j q_ij x_j + u1_i (v1 . x) + u2_i (v2 . x)) (s_i = 1 for bf16)
+    /// with the two dots from the column reduction: one fma per stored element instead of three and a scale, and no v1,
+    /// v2 re-reads per row step, which at one or two bytes per element outweighed A (bf16 pass 2 with wave-owned rows
+    /// 0.353 -> 0.324 ms at N = 32768)
+    static constexpr bool kFactored = S != DType::f32;
+    /// column chunks in flight per work-item for reduced-precision storage in the shared layout: 16 loads of 16 bytes of A
+    /// (RT rows each); f32 as tuned
     static constexpr int kChunksInFlight = (kFactored && RT < 16) ? 16 / RT : 1;
 
     struct Inputs {
@@ -865,8 +866,8 @@ public:
 
     /// The wave layout: each wave keeps kWaveRows running dots over its own rows, lane l takes the kVec columns at
     /// kVec (l + 64 s); each row's dot is one chain per lane (compensated for reduced-precision storage) and a wave
-    /// butterfly. e4m3's factored terms (v1 . x, v2 . x) are reduced over the whole work-group first, before any wave
-    /// leaves.
+    /// butterfly. The factored terms of reduced-precision storage (v1 . x, v2 . x) are reduced over the whole work-group
+    /// first, before any wave leaves.
     __device__ static void wave_rows(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
         constexpr int       W    = kWaveRows;
         const std::uint32_t rows = u32(in.A.shape()[0]), n = u32(in.A.shape()[1]);
