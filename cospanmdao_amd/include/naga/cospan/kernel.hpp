@@ -505,17 +505,17 @@ struct LocalLayout<A, Rest...> {
 /// argument).
 template <class... Arrays>
 struct LocalMem {
-    static_assert(sizeof...(Arrays) >= 1 && sizeof...(Arrays) <= 6, "cospan: a LocalMem holds one to six arrays");
-    static_assert((detail::is_local_array<Arrays>::value && ...), "cospan: a LocalMem holds cs::LocalArray types");
+    static_assert(sizeof...(Arrays) >= 1 && sizeof...(Arrays) <= 6, "cospan: LocalMem takes between 1 and 6 LocalArray declarations");
+    static_assert((detail::is_local_array<Arrays>::value && ...), "cospan: every LocalMem parameter must be a cs::LocalArray");
     static constexpr std::size_t count = sizeof...(Arrays);
     template <std::size_t I>
     using array_t = typename detail::nth_array<I, Arrays...>::type;
 
-    /// Every array empty.
+    /// All shapes zero: a stage that asks for no local memory this launch.
     CS_HD LocalMem() {
         for (range<3>& r : shapes_) r = range<3>{0, 0, 0};
     }
-    /// One shape per array, in declaration order.
+    /// The shapes, the k-th for the k-th LocalArray parameter.
     CS_HD explicit LocalMem(range<Arrays::rank>... shapes) : shapes_{detail::lift_range<Arrays::rank>(shapes)...} {}
 
     template <std::size_t I>

@@ -39,7 +39,7 @@ using cs::DType;
 
 inline constexpr int kWg = 256;  // the work-group size of every GEMVER kernel
 
-/// `v` rounded up to a multiple of `m`: a group-range launch range is a whole number of work-groups.
+/// The smallest multiple of m not below v (launch ranges of group-range components cover whole work-groups).
 inline std::size_t round_up(std::size_t v, std::size_t m) { return (v + m - 1) / m * m; }
 
 __host__ __device__ inline std::uint32_t u32(std::size_t v) { return static_cast<std::uint32_t>(v); }
