@@ -261,6 +261,16 @@ __device__ inline void ticket_wait_segments(std::uint32_t* ts, const SegmentInfo
     __syncthreads();
 }
 
+/// One poll of a shared write-back (publish_stores) by a work-group that read s0 = started after its stores had reached
+/// its XCD's L2, with d = completed and s = started read now: covered once a write-back numbered above s0 has completed
+/// (it started after those stores arrived), issue when none is in flight (the group then claims number s + 1 by a
+/// compare-and-swap on started), else wait. Host-and-device: the CPU simulation of the protocol runs this function.
+enum class WriteBackStep : std::uint8_t { covered, wait, issue };
+CS_HD inline WriteBackStep write_back_step(std::uint32_t s0, std::uint32_t d, std::uint32_t s) {
+    if (d > s0) return WriteBackStep::covered;
+    return s == d ? WriteBackStep::issue : WriteBackStep::wait;
+}
+
 /// The XCD this wave runs on (hardware register XCC_ID; work-group i of a launch lands on XCD i % 8).
 __device__ inline std::uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)); }
 
@@ -286,10 +296,10 @@ __device__ inline bool publish_stores(std::uint32_t* wb, std::uint32_t limit) {
     const std::uint32_t  s0        = poll_load(started);
     const bool           done      = bounded::poll(
         [&] {
-            const std::uint32_t d = poll_load(completed);
-            if (d > s0) return true;
-            std::uint32_t s = poll_load(started);
-            if (s != d) return false;  // one in flight: it may have started before this group's stores arrived
+            const std::uint32_t d    = poll_load(completed);
+            std::uint32_t       s    = d > s0 ? d : poll_load(started);  // covered needs no second load
+            const WriteBackStep step = write_back_step(s0, d, s);
+            if (step != WriteBackStep::issue) return step == WriteBackStep::covered;  // wait: one in flight may predate the stores
             if (!__hip_atomic_compare_exchange_strong(started, &s, s + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
                 return false;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the write-back, ordered before the count that follows

@@ -70,6 +70,17 @@ def test_policies_host_logic():
     assert "depends_on: B reads what A wrote" in out and "transient" in out
 
 
+def test_shared_write_back_protocol():
+    """The shared L2 write-back that publishes a work-group's stores in ticket mode and in the persistent batched queue
+    (kernel.hpp write_back_step, the function the device runs), simulated over 200 seeded adversarial interleavings of
+    store arrivals, polls, claims and write-back starts and completions on 1 to 4 XCDs: every published store is in
+    memory, one write-back is in flight per XCD, nothing is stuck; a planted mistake in the decision is caught."""
+    build()
+    out = run("writeback_host")
+    assert "every published store in memory" in out
+    assert "planted mistake caught in 200 of 200" in out
+
+
 def test_getting_started_demo_on_the_cpu_boundary():
     """examples/getting_started.cpp, the reference's one-component demo, included as it is and run against the CPU
     boundary with its kernel emulated by the user's own compute_forward: it prints y = 2 x + 1 for x = 1, 2, 3, 4."""
