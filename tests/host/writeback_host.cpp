@@ -45,13 +45,12 @@ struct Xcd {
     std::uint32_t     started = 0, completed = 0;
     std::set<int>     l2;              ///< stores in the L2 not yet written back
     std::set<int>     in_flight;       ///< the stores the write-back in flight took when it started
-    bool              running = false; ///< a write-back has started and not completed
 };
 
 enum class At { arrive, read_s0, poll_d, poll_s, wb_start, wb_done, set_completed, done };
 
 struct Group {
-    int           xcd = 0, jobs_left = 0, next_store = 0;
+    int           xcd = 0, jobs_left = 0;
     At            at  = At::arrive;
     std::uint32_t s0 = 0, d = 0, claimed = 0;
     std::uint64_t blocked = ~0ull;  ///< the counters' version this group saw when it decided to wait
@@ -115,14 +114,12 @@ static int simulate(int nx, int per, int jobs, const Decide& decide, std::uint64
             case At::wb_start:  // the write-back takes every dirty line of the L2
                 X.in_flight = X.l2;
                 X.l2.clear();
-                X.running = true;
-                w.at      = At::wb_done;
+                w.at = At::wb_done;
                 return true;
             case At::wb_done:  // ... and has written them to memory when the fence returns
                 memory.insert(X.in_flight.begin(), X.in_flight.end());
                 X.in_flight.clear();
-                X.running = false;
-                w.at      = At::set_completed;
+                w.at = At::set_completed;
                 return true;
             case At::set_completed:
                 X.completed = w.claimed;
