@@ -242,9 +242,10 @@ private:
             "mid", cs::connects_desc({{"bk.Pq", "rowred.Pq"}, {"bk.Ps", "cred_s.P"}}),
             cs::segment<cs::local_mem::shared>(cs::subsystem_desc<BicgkTileLocal>("bk"), cs::subsystem_desc<Rank2UpdateFromLocal>("r2u")),
             cs::global_barrier(), cs::segment(cs::subsystem_desc<RowReduce>("rowred"), cs::subsystem_desc<ColReduce<false>>("cred_s"))));
-        // G2: gtp needs y = q, so it reads B2 from global memory.
+        // G2: the partial column sums need y = q, so they read B2 from global memory: the two-pass form's pass 1 with zero
+        // rank-2 vectors (fma(0, 0, b) = b: GemvTPartial's fma chain over each strip, bit for bit, at the streaming rate).
         add_subsystem(cs::fuse_components("g2", cs::connects_desc({{"gtp.P", "cred.P"}, {"cred.x", "gemv.x"}}),
-                                          cs::subsystem_desc<GemvTPartial>("gtp"), cs::global_barrier(),
+                                          cs::subsystem_desc<AhatTPartial<DType::f32>>("gtp", kRt), cs::global_barrier(),
                                           cs::subsystem_desc<ColReduce<true>>("cred"), cs::global_barrier(), cs::subsystem_desc<Gemv>("gemv")));
         tie_inputs<DType::f32, 1>({"g1.cred.scale", "g1.gemv.scale"}, "g1_scale", cs::shape(1));
         tie_inputs<DType::f32, 1>({"g2.cred.scale", "g2.gemv.scale"}, "g2_scale", cs::shape(1));
@@ -252,7 +253,7 @@ private:
         connect("g1.r2u.B", "mid.bk.B");      // B1
         connect("g1.cred.x", "mid.bk.p");     // p = x1
         connect("g1.gemv.y", "mid.bk.r");     // r = w1
-        connect("mid.r2u.B", "g2.gtp.B");     // B2 feeds both of G2's readers
+        connect("mid.r2u.B", "g2.gtp.A");     // B2 feeds both of G2's readers
         connect("mid.r2u.B", "g2.gemv.B");
         connect("mid.rowred.q", "g2.gtp.y");  // y = q
         connect("mid.cred_s.x", "g2.cred.z"); // z = s
@@ -285,6 +286,8 @@ private:
         init_input("g1_scale", cs::init::fill(ab));
         init_input("mid_scale", cs::init::fill(1.0f, {1}));
         init_input("g2_scale", cs::init::fill(ab));
+        init_input("g2.gtp.row_scale", cs::init::fill(1.0f, {1}));
+        for (const char* v : {"g2.gtp.u1", "g2.gtp.v1", "g2.gtp.u2", "g2.gtp.v2"}) init_input(v, cs::init::fill(0.0f, {n}));
     }
 
     Style              style_;
