@@ -195,7 +195,8 @@ __device__ inline F4* tile_of(const cs::GroupItem<1>& it, TileMem lm) {
 }
 }  // namespace detail
 
-/// B = A + u1 v1^T + u2 v2^T for the tile; B goes to global memory (B1 is an output) and stays in the tile.
+/// B = A + u1 v1^T + u2 v2^T for the tile; B goes to global memory (B1 is an output) and stays in the tile. B is
+/// written with nontemporal stores: only a later launch reads it back, so it need not stay in the caches.
 class Rank2UpdateLocal : public cs::Component<Rank2UpdateLocal> {
 public:
     static constexpr int max_work_group_size = kWg;
@@ -225,8 +226,8 @@ public:
             F4 b;
 #pragma unroll
             for (int e = 0; e < 4; ++e) b[e] = rank2(a[k][e], u1[k], v1[e], u2[k], v2[e]);
-            *reinterpret_cast<F4*>(B + k * ldb) = b;
-            tile[k * kWg + tid]                 = b;
+            __builtin_nontemporal_store(b, reinterpret_cast<F4*>(B + k * ldb));
+            tile[k * kWg + tid] = b;
         }
     }
 
@@ -375,7 +376,7 @@ public:
             F4       b;
 #pragma unroll
             for (int e = 0; e < 4; ++e) b[e] = rank2(a[e], u1[k], v1[e], u2[k], v2[e]);
-            *reinterpret_cast<F4*>(B + k * ldb) = b;
+            __builtin_nontemporal_store(b, reinterpret_cast<F4*>(B + k * ldb));
         }
     }
 
