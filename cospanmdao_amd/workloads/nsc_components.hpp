@@ -249,6 +249,7 @@ public:
         __syncthreads();
 
         // ---- the cells: R, the local timestep, the wave speed, the squares ----
+        // R goes out with nontemporal stores: no stage of the run reads it back.
         float* const Rout = cs::raw::data(out.R);
         float        dt = std::numeric_limits<float>::infinity(), ws = -std::numeric_limits<float>::infinity();
         float        sq[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -259,7 +260,7 @@ public:
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const float R = Rc[k][q] + (Fl[q * FS + r * kTX + c] - Fl[q * FS + (r + 1) * kTX + c]);
-                Rout[q * plane + o] = R;
+                __builtin_nontemporal_store(R, Rout + (q * plane + o));
                 sq[q] += R * R;
             }
             const int   e  = (r + kHalo) * PW + c + kHalo;
