@@ -153,6 +153,25 @@ def test_wide_instances(mode):
 
 
 @pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("R,RT,G", [(32, 4, 7), (64, 32, 48)])
+def test_tiny_and_odd_instances(R, RT, G, mode):
+    """48 instances of 1 to 40 rows (seed 7: sixteen of them 1, 2 or 3 rows): rows shorter than one 4-column chunk, the
+    masked last chunk of pass 1, pass 2's scalar column tail, strips and pass-2 groups with fewer rows than they hold."""
+    print("batch tiny", R, RT, G, "mode", mode, flush=True)
+    b = Batch(48, 1, 40, seed=7, mode=mode)
+    try:
+        assert int(b.sizes.min()) == 1 and int((b.sizes < 4).sum()) == 16, b.sizes
+        b.params(R, RT, G)
+        b.run()
+        b.run()
+        counts = b.read("counts", np.uint32, 9 * 48)
+        assert np.array_equal(counts, counts_of(b.sizes, R, RT))
+        check_instances(b, range(48), R)
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
 def test_replays_and_parameter_changes(mode):
     print("batch replays, mode", mode, flush=True)
     b = Batch(40, 64, 1200, seed=2, mode=mode)
