@@ -360,7 +360,7 @@ public:
     };
     /// Up to kOneWave partials (the fused tile's), one wave does it all and the other waves return at once: lane l sums
     /// partials l, l + 64, ... in order, then a butterfly over the lanes; no LDS, no barrier. More partials (the unfused
-    /// variant's one per 256 cells): every work-item sums a stride of 256, each wave reduces by butterfly, and the four
+    /// variant's one per 4096 cells): every work-item sums a stride of 256, each wave reduces by butterfly, and the four
     /// wave results meet through LDS after one barrier, summed pairwise in wave order. Both orders are fixed.
     static constexpr std::uint32_t kOneWave = 1024;
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
@@ -497,10 +497,14 @@ public:
     std::unique_ptr<cs::LaunchDescription> define_forward() override { return this->create_launch_config(cs::range_desc(inputs.u.shape())); }
 };
 
-/// Per work-group of kWg cells: the FP64 sums of squares of Ru, Rv, Rc (the partials NsNorm reduces).
+/// Per work-group of kCells cells: the FP64 sums of squares of Ru, Rv, Rc (the partials NsNorm reduces). Work-item l of
+/// group g sums cells g kCells + k kWg + l for k = 0 .. kPerItem-1 in that order (each step one coalesced run of the
+/// group), then the group reduces: one reduction per 4096 cells instead of per 256 (0.19 -> 0.05 ms at 4096^2).
 class NsSquares : public cs::Component<NsSquares> {
 public:
-    static constexpr int max_work_group_size = kWg;
+    static constexpr int         max_work_group_size = kWg;
+    static constexpr std::size_t kPerItem            = 16;
+    static constexpr std::size_t kCells              = kPerItem * kWg;
     struct Inputs {
         cs::Input<DType::f32, 2> Ru, Rv, Rc;
     };
@@ -508,15 +512,20 @@ public:
         cs::Output<DType::f64, 2> part;
     };
     __device__ static void compute_forward(cs::GroupItem<1> it, const Inputs& in, const Outputs& out) {
-        const std::size_t n = in.Ru.size(), t = it.get_global_id(0);
+        const std::size_t n = in.Ru.size(), t0 = it.get_group(0) * kCells + it.get_local_id(0);
         const float*      a = cs::raw::data(in.Ru);
         const float*      b = cs::raw::data(in.Rv);
         const float*      c = cs::raw::data(in.Rc);
         double            s0 = 0.0, s1 = 0.0, s2 = 0.0;
-        if (t < n) {
-            s0 = static_cast<double>(a[t]) * a[t];
-            s1 = static_cast<double>(b[t]) * b[t];
-            s2 = static_cast<double>(c[t]) * c[t];
+#pragma unroll
+        for (std::size_t k = 0; k < kPerItem; ++k) {
+            const std::size_t t = t0 + k * kWg;
+            if (t < n) {
+                const float x = __builtin_nontemporal_load(a + t), y = __builtin_nontemporal_load(b + t), z = __builtin_nontemporal_load(c + t);
+                s0 += static_cast<double>(x) * x;
+                s1 += static_cast<double>(y) * y;
+                s2 += static_cast<double>(z) * z;
+            }
         }
         const auto g = it.get_group();
         s0           = cs::reduce_over_group(g, s0, cs::plus<double>());
@@ -544,7 +553,7 @@ public:
     }
 
 private:
-    static std::size_t groups(std::size_t n) { return (n + kWg - 1) / kWg; }
+    static std::size_t groups(std::size_t n) { return (n + kCells - 1) / kCells; }
 };
 
 CS_COMPONENTS_END

@@ -69,9 +69,10 @@ def compare(g, store, fields):
 @pytest.mark.parametrize("style", [FUSED, UNFUSED])
 # nx = 8: two live lanes, both halos the periodic wrap; nx = 1032: the last wave of a row holds two live lanes; ty = 7 and
 # ty = 64 with a partial last strip: the five-row steady state, its tail, and a tile shorter than the steady state;
-# 1024 x 520: the unfused variant's 2080 partials take the norm's four-wave path (up to 1024 it is one wave)
+# 1032 x 4072: the unfused variant's 1026 partials (one per 4096 cells, the last one partial) take the norm's four-wave
+# path (up to 1024 it is one wave)
 @pytest.mark.parametrize("nx,ny,ty", [(64, 48, 16), (1024, 40, 16), (2048, 64, 7), (1032, 33, 32), (8, 24, 16), (1032, 130, 64),
-                                      (4096, 12, 4), (1024, 520, 16)])
+                                      (4096, 12, 4), (1024, 520, 16), (1032, 4072, 16)])
 def test_ns_small_against_fp64(style, nx, ny, ty):
     print("ns", style, nx, ny, ty, flush=True)
     g = Ns(style, nx, ny, ty=ty, seed=3)
