@@ -409,6 +409,7 @@ public:
 // =============================================================================
 
 /// One index per cell: the eight derivative fields (u_x, u_y, v_x, v_y, p_x, p_y, lap u, lap v) as [8, ny, nx].
+/// Written with nontemporal stores, as are NsCombine's residuals: each is read back only by a later launch.
 template <DType S>
 class NsGradients : public cs::Component<NsGradients<S>> {
 public:
@@ -429,14 +430,14 @@ public:
         const float uc = f(in.u, i, j), ue = f(in.u, i, jp), uw = f(in.u, i, jm), un = f(in.u, ip, j), us = f(in.u, im, j);
         const float vc = f(in.v, i, j), ve = f(in.v, i, jp), vw = f(in.v, i, jm), vn = f(in.v, ip, j), vs = f(in.v, im, j);
         const float pe = f(in.p, i, jp), pw = f(in.p, i, jm), pn = f(in.p, ip, j), ps = f(in.p, im, j);
-        out.d[cs::id<3>{0, i, j}] = (ue - uw) * k.inv2h;
-        out.d[cs::id<3>{1, i, j}] = (un - us) * k.inv2h;
-        out.d[cs::id<3>{2, i, j}] = (ve - vw) * k.inv2h;
-        out.d[cs::id<3>{3, i, j}] = (vn - vs) * k.inv2h;
-        out.d[cs::id<3>{4, i, j}] = (pe - pw) * k.inv2h;
-        out.d[cs::id<3>{5, i, j}] = (pn - ps) * k.inv2h;
-        out.d[cs::id<3>{6, i, j}] = (((ue - uc) + (uw - uc)) + ((un - uc) + (us - uc))) * k.invh2;
-        out.d[cs::id<3>{7, i, j}] = (((ve - vc) + (vw - vc)) + ((vn - vc) + (vs - vc))) * k.invh2;
+        __builtin_nontemporal_store((ue - uw) * k.inv2h, &out.d[cs::id<3>{0, i, j}]);
+        __builtin_nontemporal_store((un - us) * k.inv2h, &out.d[cs::id<3>{1, i, j}]);
+        __builtin_nontemporal_store((ve - vw) * k.inv2h, &out.d[cs::id<3>{2, i, j}]);
+        __builtin_nontemporal_store((vn - vs) * k.inv2h, &out.d[cs::id<3>{3, i, j}]);
+        __builtin_nontemporal_store((pe - pw) * k.inv2h, &out.d[cs::id<3>{4, i, j}]);
+        __builtin_nontemporal_store((pn - ps) * k.inv2h, &out.d[cs::id<3>{5, i, j}]);
+        __builtin_nontemporal_store((((ue - uc) + (uw - uc)) + ((un - uc) + (us - uc))) * k.invh2, &out.d[cs::id<3>{6, i, j}]);
+        __builtin_nontemporal_store((((ve - vc) + (vw - vc)) + ((vn - vc) + (vs - vc))) * k.invh2, &out.d[cs::id<3>{7, i, j}]);
     }
 
     Inputs  inputs;
@@ -472,9 +473,9 @@ public:
         const Coef        k  = coef_of(in.coef);
         const float       uc = Field<S>::one(in.u[c]), vc = Field<S>::one(in.v[c]);
         auto              d  = [&](std::size_t q) { return in.d[cs::id<3>{q, i, j}]; };
-        out.Ru[c]            = fmaf(-k.nu, d(6), fmaf(uc, d(0), fmaf(vc, d(1), d(4))));
-        out.Rv[c]            = fmaf(-k.nu, d(7), fmaf(uc, d(2), fmaf(vc, d(3), d(5))));
-        out.Rc[c]            = d(0) + d(3);
+        __builtin_nontemporal_store(fmaf(-k.nu, d(6), fmaf(uc, d(0), fmaf(vc, d(1), d(4)))), &out.Ru[c]);
+        __builtin_nontemporal_store(fmaf(-k.nu, d(7), fmaf(uc, d(2), fmaf(vc, d(3), d(5)))), &out.Rv[c]);
+        __builtin_nontemporal_store(d(0) + d(3), &out.Rc[c]);
     }
 
     Inputs  inputs;
